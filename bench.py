@@ -1,0 +1,253 @@
+"""Benchmark: effective GFLOP/s (2 n^3 / t) of the Strassen-Winograd multiply
+at n = 8192 on one MI355X, split precision (INT8 residues, CRT) against plain
+FP64, with the accuracy gates of the metric.
+
+Workload (one "step" = one multiply of device-resident CAPS-layout inputs):
+  n = 8192, nrec = 2, pattern "bb" (two batched levels: 49 leaf products of
+  2048^2 in one launch), bs = 1; A, B entries u * 2^e, u uniform [-1, 1),
+  e uniform integer in [-10, 10] (fixed seeds).  The split run is the
+  reported value; the plain FP64 run of the same multiply is reported beside
+  it with the ratio.  After timing:
+    * normwise error max|C - C*| / max|C*| on sampled rows against the
+      compensated (Dot2) classical product from the CPU oracle,
+    * an integer pair in [-64, 64] through the split path checked exactly
+      (Freivalds' test over all of C plus sampled rows, exact in FP64).
+The roofline entry is for the leaf GEMM launch (INT8 MFMA), timed live with
+HIP events on the launch stream through the library's phase timers.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--n 8192]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_INT8_TOPS = 5000.0   # MI355X dense INT8 (TOP/s)
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 matrix
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(torch, n, seed, integer=False):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    if integer:
+        return torch.randint(-64, 65, (n * n,), generator=g, device="cuda",
+                             dtype=torch.int32).to(torch.float64)
+    u = torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    e = torch.randint(-10, 11, (n * n,), generator=g, device="cuda", dtype=torch.int32)
+    return torch.ldexp(u, e.to(torch.float64))
+
+
+def time_steps(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def leaf_kernel_ms(caps_amd, torch, dA, dB, dC, desc, pattern, mode, moduli, reps=3):
+    """Average duration of the leaf GEMM launch, HIP events on its stream."""
+    caps_amd.reset_counters()
+    for _ in range(reps):
+        caps_amd.multiply_device(dA, dB, dC, desc, pattern, mode=mode, moduli=moduli,
+                                 flags=caps_amd.FLAG_TIMERS)
+    torch.cuda.synchronize()
+    c = caps_amd.counters()
+    t, k = c["timers_ms"], c["timed_launches"]
+    per = {name: (t[name] / k[name] if k[name] else 0.0) for name in t}
+    per_call = {name: t[name] / reps for name in t}
+    return per, per_call
+
+
+def sampled_error(oracle, A_cm, B_cm, C_cm, rows):
+    n = A_cm.shape[0]
+    rr = np.repeat(rows, n).astype(np.int32)
+    cc = np.tile(np.arange(n), len(rows)).astype(np.int32)
+    ref = oracle.dot2_entries(A_cm, B_cm, rr, cc).reshape(len(rows), n)
+    got = C_cm[rows, :]
+    return float(np.abs(got - ref).max() / np.abs(ref).max())
+
+
+def exact_integer_check(A_cm, B_cm, C_cm, rows, seed=11):
+    """Freivalds over all of C plus exact sampled rows.  Entries <= 64, K <=
+    8192: every partial sum stays below 2^53, so FP64 BLAS is exact here."""
+    rng = np.random.default_rng(seed)
+    x = rng.integers(-8, 9, (A_cm.shape[0], 2)).astype(np.float64)
+    lhs = C_cm @ x
+    rhs = A_cm @ (B_cm @ x)
+    ok = bool(np.array_equal(lhs, rhs))
+    ok &= bool(np.array_equal(C_cm[rows, :], A_cm[rows, :] @ B_cm))
+    return ok
+
+
+def cpu_baseline(oracle, n=4096, r=2, threads=None):
+    """The oracle's Winograd restatement (multiply.cpp:409-454 order, blocked
+    leaf dgemm) on the host cores: one multiply at n=4096, nrec=2, 'dd'."""
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    g = oracle.Drand48()
+    a = g.uniform(n * n)
+    b = g.uniform(n * n)
+    t0 = time.perf_counter()
+    oracle.multiply(a, b, n, r, 1, "d" * r, threads)
+    t = time.perf_counter() - t0
+    return {"value": round(2.0 * n ** 3 / t / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
+            "kind": "port",
+            "sample": f"one oracle multiply n={n} nrec={r} pattern 'dd' bs=1, uniform [-1,1) "
+                      f"from drand48, {threads} OpenMP threads, {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--nrec", type=int, default=2)
+    ap.add_argument("--pattern", default="bb")
+    ap.add_argument("--moduli", type=int, default=14)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU BFS exchange is a design, not part of this build)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import caps_amd
+    import oracle
+
+    if not os.path.exists(caps_amd.LIB_PATH):
+        from caps_amd import build
+        build.build()
+    torch.cuda.set_device(0)
+    caps_amd.load()
+    n, r, pat, K = args.n, args.nrec, args.pattern, args.moduli
+    desc = caps_amd.MatDescriptor.one_device(n, r, 1)
+    eff = 2.0 * n ** 3
+
+    log(f"[bench] inputs n={n}")
+    dA = make_inputs(torch, n, 1234)
+    dB = make_inputs(torch, n, 5678)
+    dC_split = torch.empty_like(dA)
+    dC_plain = torch.empty_like(dA)
+
+    def run_split():
+        caps_amd.multiply_device(dA, dB, dC_split, desc, pat, mode=caps_amd.MODE_OZAKI_INT8,
+                                 moduli=K)
+
+    def run_plain():
+        caps_amd.multiply_device(dA, dB, dC_plain, desc, pat, mode=caps_amd.MODE_FP64)
+
+    log("[bench] split precision (INT8 residues, CRT)")
+    t_split = time_steps(torch, run_split, args.steps, args.warmup)
+    log(f"[bench] split {t_split * 1e3:.3f} ms/step")
+    log("[bench] plain FP64")
+    t_plain = time_steps(torch, run_plain, args.steps, args.warmup)
+    log(f"[bench] plain {t_plain * 1e3:.3f} ms/step")
+
+    # roofline of the dominant kernel: the leaf GEMM launch (all 49 x K INT8 GEMMs)
+    per_launch, per_call = leaf_kernel_ms(caps_amd, torch, dA, dB, dC_split, desc, pat,
+                                          caps_amd.MODE_OZAKI_INT8, K)
+    m = n >> r
+    leaves = 7 ** r
+    int_ops = K * 2.0 * leaves * m ** 3
+    gemm_ms = per_launch["mul"]
+    achieved = int_ops / (gemm_ms * 1e-3) / 1e12
+    plain_launch, plain_call = leaf_kernel_ms(caps_amd, torch, dA, dB, dC_plain, desc, pat,
+                                              caps_amd.MODE_FP64, 0)
+    f64_ach = 2.0 * leaves * m ** 3 / (plain_launch["mul"] * 1e-3) / 1e12
+    log(f"[bench] split phases per call (ms): {per_call}")
+    log(f"[bench] plain phases per call (ms): {plain_call}")
+
+    result = {
+        "metric": "effective GFLOP/s (2n^3/t) at n=8192, err<=1e-14; split-precision vs plain FP64",
+        "value": round(eff / t_split / 1e9, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_split * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 in/out, int8 MFMA leaf products",
+        "data": "synthetic: entries u*2^e, u~U[-1,1), e~U{-10..10}, fixed seeds",
+        "config": {"workload": f"CAPS multiply n={n} nrec={r} pattern '{pat}' bs=1, "
+                               f"{leaves} leaf products of {m}^2 per launch",
+                   "n": n, "nrec": r, "pattern": pat, "moduli": K,
+                   "plain_fp64_gflops": round(eff / t_plain / 1e9, 2),
+                   "plain_fp64_ms_per_step": round(t_plain * 1e3, 4),
+                   "split_over_plain": round(t_plain / t_split, 3),
+                   "phases_ms_split": {k: round(v, 4) for k, v in per_call.items() if v},
+                   "phases_ms_plain": {k: round(v, 4) for k, v in plain_call.items() if v},
+                   "plain_leaf_gemm_tflops": round(f64_ach, 2),
+                   "plain_leaf_gemm_frac_fp64_peak": round(f64_ach / PEAK_FP64_TFLOPS, 4)},
+        "roofline": {"bound": "mfma", "kernel": "oz_igemm_kernel (INT8 MFMA, all leaves x moduli)",
+                     "achieved": round(achieved, 2), "peak": PEAK_INT8_TOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_INT8_TOPS, 4),
+                     "traffic": None,
+                     "note": "INT8 multiply-adds counted as 2 ops; per launch "
+                             f"{int_ops:.4e} ops = moduli x 2 x 49 x 2048^3, "
+                             f"{gemm_ms:.3f} ms"},
+    }
+
+    if not args.no_check:
+        log("[bench] accuracy: copying operands to host")
+        A_cm = oracle.from_caps(dA.cpu().numpy(), n, r, 1)
+        B_cm = oracle.from_caps(dB.cpu().numpy(), n, r, 1)
+        rng = np.random.default_rng(99)
+        rows = np.sort(rng.choice(n, 16, replace=False))
+        err_split = sampled_error(oracle, A_cm, B_cm,
+                                  oracle.from_caps(dC_split.cpu().numpy(), n, r, 1), rows)
+        err_plain = sampled_error(oracle, A_cm, B_cm,
+                                  oracle.from_caps(dC_plain.cpu().numpy(), n, r, 1), rows)
+        log(f"[bench] error split {err_split:.3e} plain {err_plain:.3e}")
+        del A_cm, B_cm
+        log("[bench] integer pair through the split path")
+        iA = make_inputs(torch, n, 4321, integer=True)
+        iB = make_inputs(torch, n, 8765, integer=True)
+        iC = torch.empty_like(iA)
+        caps_amd.multiply_device(iA, iB, iC, desc, pat, mode=caps_amd.MODE_OZAKI_INT8, moduli=K)
+        torch.cuda.synchronize()
+        exact = exact_integer_check(oracle.from_caps(iA.cpu().numpy(), n, r, 1),
+                                    oracle.from_caps(iB.cpu().numpy(), n, r, 1),
+                                    oracle.from_caps(iC.cpu().numpy(), n, r, 1), rows)
+        log(f"[bench] integer pair bit-exact: {exact}")
+        result["config"]["error_split"] = err_split
+        result["config"]["error_plain"] = err_plain
+        result["config"]["error_metric"] = ("max|C-C*|/max|C*| over 16 sampled full rows, "
+                                            "C* = Dot2 compensated classical product")
+        result["config"]["integer_pair_bit_exact"] = exact
+        result["config"]["meets_error_bar"] = err_split <= 1e-14
+
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle restatement)")
+        result["cpu_baseline"] = cpu_baseline(oracle)
+    else:
+        result["cpu_baseline"] = None
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,270 @@
+"""caps_amd: MI355X (gfx950) Strassen-Winograd multiply with CAPS's interface.
+
+Python mirror of the reference's entry points over the C ABI in
+include/caps_amd.h (libcaps_amd.so, loaded with ctypes):
+
+    MatDescriptor           struct MatDescriptor (matrix.h:8-16)
+    multiply(A, B, C, desc, pattern=None)
+                            void multiply(...) (multiply.h:13)
+    verify_descriptor(desc) verifyDescriptor (matrix.cpp:3-7), raises
+    num_entries_per_proc    numEntriesPerProc (matrix.cpp:9-12)
+    distribute_from_1proc / collect_to_1proc
+                            distributeFrom1Proc / collectTo1Proc with one
+                            process (communication.cpp:657-659, 706-708)
+    print_counters          printCounters (counters.cpp:61-74)
+
+Host arrays are numpy float64; device arrays are torch tensors on a HIP
+device (torch is only used for device memory and streams).  The library has
+no CPU fallback: if libcaps_amd.so is missing or cannot be loaded, every call
+raises CapsError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcaps_amd.so")
+
+SEVEN = 7  # library.h:6
+
+MODE_FP64 = 0
+MODE_OZAKI_INT8 = 1
+MODE_AUTO = 99
+FLAG_NO_GRAPH = 1
+FLAG_REFERENCE_AUTO = 2
+FLAG_TIMERS = 4
+DEFAULT_MODULI = 14
+# phase timers (counters.h:13-18 order, then the split mode's phases)
+TIMER_NAMES = ("comm", "add", "mul", "rearrange", "comm_summa", "split", "recon", "unused")
+NUM_TIMERS = 8
+
+_STATUS = {0: "OK", 1: "EINVAL", 2: "EPATTERN", 3: "ENOMEM", 4: "EHIP", 5: "ENODEV",
+           6: "EUNSUPPORTED"}
+
+
+class CapsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"caps error {_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class MatDescriptor(ctypes.Structure):
+    """struct MatDescriptor (matrix.h:8-16); DESC_SIZE = 7 ints."""
+    _fields_ = [("lda", ctypes.c_int), ("nrec", ctypes.c_int), ("nproc", ctypes.c_int),
+                ("nprocr", ctypes.c_int), ("nprocc", ctypes.c_int),
+                ("nproc_summa", ctypes.c_int), ("bs", ctypes.c_int)]
+
+    @classmethod
+    def one_device(cls, lda: int, nrec: int, bs: int = 1) -> "MatDescriptor":
+        return cls(lda, nrec, 1, 1, 1, 1, bs)
+
+
+DESC_SIZE = 7
+assert ctypes.sizeof(MatDescriptor) == DESC_SIZE * ctypes.sizeof(ctypes.c_int)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("moduli", ctypes.c_int), ("flags", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("additions", ctypes.c_longlong), ("multiplications", ctypes.c_longlong),
+                ("leaf_products", ctypes.c_longlong), ("peak_workspace_mb", ctypes.c_double),
+                ("pattern", ctypes.c_char * 256),
+                ("timers_ms", ctypes.c_double * 8), ("timed_launches", ctypes.c_longlong * 8)]
+
+
+# every symbol include/caps_amd.h declares, with (restype, argtypes)
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_D = ctypes.POINTER(MatDescriptor)
+_O = ctypes.POINTER(Options)
+SIGNATURES = {
+    "caps_init": (ctypes.c_int, [ctypes.c_int]),
+    "caps_last_error": (ctypes.c_char_p, []),
+    "caps_verify_desc": (ctypes.c_int, [_D]),
+    "caps_num_entries": (ctypes.c_longlong, [_D]),
+    "caps_multiply": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_char_p, _O]),
+    "caps_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_char_p, _O, _vp]),
+    "caps_to_layout": (ctypes.c_int, [_vp, _vp, _D]),
+    "caps_from_layout": (ctypes.c_int, [_vp, _vp, _D]),
+    "caps_to_layout_device": (ctypes.c_int, [_vp, _vp, _D, _vp]),
+    "caps_from_layout_device": (ctypes.c_int, [_vp, _vp, _D, _vp]),
+    "caps_get_counters": (ctypes.c_int, [ctypes.POINTER(Counters)]),
+    "caps_reset_counters": (ctypes.c_int, []),
+    "caps_set_available_memory": (ctypes.c_int, [ctypes.c_longlong]),
+    "caps_release": (ctypes.c_int, []),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libcaps_amd.so (raises CapsError if it is absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise CapsError(6, f"{path} not built; run `python -m caps_amd.build` "
+                           "(there is no CPU fallback)")
+    try:
+        lib = ctypes.CDLL(path)
+    except OSError as e:
+        raise CapsError(6, f"cannot load {path}: {e}") from e
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise CapsError(rc, load().caps_last_error().decode(errors="replace"))
+
+
+def _opts(mode: int, moduli: int, flags: int) -> Options:
+    return Options(mode, moduli, flags, 0)
+
+
+def _pattern(p):
+    if p is None:
+        return None
+    return p.encode() if isinstance(p, str) else bytes(p)
+
+
+# ------------------------------------------------------------------ descriptor
+def verify_descriptor(desc: MatDescriptor) -> None:
+    _check(load().caps_verify_desc(ctypes.byref(desc)))
+
+
+def num_entries_per_proc(desc: MatDescriptor) -> int:
+    return int(load().caps_num_entries(ctypes.byref(desc)))
+
+
+# ------------------------------------------------------------------ multiply
+def _host_ptr(a: np.ndarray, n2: int, writable: bool = False):
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise CapsError(1, "host arrays must be C-contiguous float64 numpy arrays")
+    if a.size != n2:
+        raise CapsError(1, f"array has {a.size} entries, descriptor needs {n2}")
+    if writable and not a.flags["WRITEABLE"]:
+        raise CapsError(1, "output array is read-only")
+    return a.ctypes.data
+
+
+def multiply(A: np.ndarray, B: np.ndarray, C: np.ndarray, desc: MatDescriptor,
+             pattern: str | None = None, mode: int = MODE_FP64, moduli: int = 0,
+             flags: int = 0) -> None:
+    """C = A * B on the device, all three in the CAPS layout (host memory).
+    multiply.h:13: A and B are read, C is overwritten."""
+    n2 = desc.lda * desc.lda
+    o = _opts(mode, moduli, flags)
+    _check(load().caps_multiply(_host_ptr(A, n2), _host_ptr(B, n2), _host_ptr(C, n2, True),
+                                ctypes.byref(desc), _pattern(pattern), ctypes.byref(o)))
+
+
+def _dev_ptr(t, n2: int) -> int:
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        raise CapsError(1, "device arrays must be float64 torch tensors on the GPU")
+    if not t.is_contiguous() or t.numel() != n2:
+        raise CapsError(1, f"device array must be contiguous with {n2} entries")
+    return t.data_ptr()
+
+
+def _stream_ptr(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def multiply_device(A, B, C, desc: MatDescriptor, pattern: str | None = None,
+                    mode: int = MODE_FP64, moduli: int = 0, flags: int = 0,
+                    stream=None) -> None:
+    """Enqueue C = A * B on `stream` (default: torch's current stream) for
+    device-resident CAPS-layout tensors.  Does not synchronise."""
+    n2 = desc.lda * desc.lda
+    o = _opts(mode, moduli, flags)
+    _check(load().caps_multiply_device(_dev_ptr(A, n2), _dev_ptr(B, n2), _dev_ptr(C, n2),
+                                       ctypes.byref(desc), _pattern(pattern), ctypes.byref(o),
+                                       _stream_ptr(stream)))
+
+
+# ------------------------------------------------------------------ layout
+def distribute_from_1proc(desc: MatDescriptor, colmajor: np.ndarray) -> np.ndarray:
+    """Column-major n x n (numpy, indexed [i, j]) -> CAPS-layout vector."""
+    n = desc.lda
+    src = np.ascontiguousarray(np.asfortranarray(colmajor, dtype=np.float64).ravel(order="F"))
+    out = np.empty(n * n, dtype=np.float64)
+    _check(load().caps_to_layout(_host_ptr(src, n * n), _host_ptr(out, n * n, True),
+                                 ctypes.byref(desc)))
+    return out
+
+
+def collect_to_1proc(desc: MatDescriptor, caps: np.ndarray) -> np.ndarray:
+    """CAPS-layout vector -> n x n numpy matrix."""
+    n = desc.lda
+    src = np.ascontiguousarray(caps, dtype=np.float64)
+    out = np.empty(n * n, dtype=np.float64)
+    _check(load().caps_from_layout(_host_ptr(src, n * n), _host_ptr(out, n * n, True),
+                                   ctypes.byref(desc)))
+    return out.reshape((n, n), order="F")
+
+
+def to_layout_device(colmajor, caps, desc: MatDescriptor, stream=None) -> None:
+    n2 = desc.lda * desc.lda
+    _check(load().caps_to_layout_device(_dev_ptr(colmajor, n2), _dev_ptr(caps, n2),
+                                        ctypes.byref(desc), _stream_ptr(stream)))
+
+
+def from_layout_device(caps, colmajor, desc: MatDescriptor, stream=None) -> None:
+    n2 = desc.lda * desc.lda
+    _check(load().caps_from_layout_device(_dev_ptr(caps, n2), _dev_ptr(colmajor, n2),
+                                          ctypes.byref(desc), _stream_ptr(stream)))
+
+
+# ------------------------------------------------------------------ counters / memory
+def counters() -> dict:
+    c = Counters()
+    _check(load().caps_get_counters(ctypes.byref(c)))
+    return {"additions": c.additions, "multiplications": c.multiplications,
+            "leaf_products": c.leaf_products, "peak_workspace_mb": c.peak_workspace_mb,
+            "pattern": c.pattern.decode(),
+            "timers_ms": {TIMER_NAMES[k]: c.timers_ms[k] for k in range(NUM_TIMERS)},
+            "timed_launches": {TIMER_NAMES[k]: c.timed_launches[k] for k in range(NUM_TIMERS)}}
+
+
+def reset_counters() -> None:
+    _check(load().caps_reset_counters())
+
+
+def set_available_memory(doubles: int) -> None:
+    """setAvailableMemory (memory.cpp:31-33), in doubles."""
+    _check(load().caps_set_available_memory(int(doubles)))
+
+
+def release() -> None:
+    _check(load().caps_release())
+
+
+def print_counters(t: float, desc: MatDescriptor) -> str:
+    """The report of printCounters (counters.cpp:61-74) for one device."""
+    c = counters()
+    flops = c["additions"] + c["multiplications"]
+    lda = desc.lda
+    lines = [
+        f"peak memory usage {c['peak_workspace_mb']:.2f} MB",
+        f"Execution pattern {c['pattern']}",
+        f"flops={flops} ({c['additions'] - c['multiplications']}+{2 * c['multiplications']})",
+        f"time {t:f} actual Gflop/s {flops / 1e9 / t:f} "
+        f"effective Gflop/s {2.0 * lda ** 3 / 1e9 / t:f}",
+    ]
+    text = "\n".join(lines)
+    print(text)
+    return text

@@ -1,0 +1,456 @@
+// C ABI of the library (include/caps_amd.h): argument checking, pattern
+// resolution, the plan cache, hipGraph replay and host-pointer staging.
+//
+// Errors never abort: every entry point returns a caps_status and records a
+// message for caps_last_error() (the reference prints and exits instead,
+// multiply.cpp:94-104).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/caps_amd.h"
+#include "engine.h"
+#include "ozaki.h"
+#include "planner.h"
+
+namespace caps {
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(CAPS_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));             \
+  } while (0)
+
+struct State {
+  std::mutex mu;
+  int device = -1;
+  int cus = 256;
+  std::vector<std::unique_ptr<Plan>> plans;
+  unsigned long long clock = 0;
+  long long additions = 0, multiplications = 0, leaves = 0;
+  double peak_mb = 0;
+  std::string exec_pattern;
+  double timers_ms[CAPS_NUM_TIMERS] = {};
+  long long timed[CAPS_NUM_TIMERS] = {};
+  long long available_doubles = 1LL << 28;  // memory.cpp:4
+  // host staging buffers (device side) for the host-pointer entry points
+  double *stage[3] = {nullptr, nullptr, nullptr};
+  size_t stage_elems = 0;
+  hipStream_t stage_stream = nullptr;
+};
+
+State &st() {
+  static State s;
+  return s;
+}
+
+constexpr size_t MAX_PLANS = 24;
+
+int ensure_device_locked(State &s) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (s.device == dev) return CAPS_OK;
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(CAPS_ENODEV, "device %d is %s, this library is built for gfx950", dev,
+                prop.gcnArchName);
+  s.device = dev;
+  s.cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  return CAPS_OK;
+}
+
+void free_plan(Plan &p) {
+  if (p.graph) hipGraphExecDestroy(p.graph);
+  if (p.capture_stream) hipStreamDestroy(p.capture_stream);
+  if (p.d_tables) hipFree(p.d_tables);
+  if (p.d_work) hipFree(p.d_work);
+  p.graph = nullptr;
+  p.capture_stream = nullptr;
+  p.d_tables = p.d_work = nullptr;
+}
+
+void evict_locked(State &s, size_t keep) {
+  if (s.plans.size() <= keep) return;
+  hipDeviceSynchronize();
+  std::sort(s.plans.begin(), s.plans.end(),
+            [](const std::unique_ptr<Plan> &a, const std::unique_ptr<Plan> &b) {
+              return a->last_use > b->last_use;
+            });
+  while (s.plans.size() > keep) {
+    free_plan(*s.plans.back());
+    s.plans.pop_back();
+  }
+}
+
+int check_desc(const caps_desc *d) {
+  if (!d) return fail(CAPS_EINVAL, "desc is NULL");
+  if (d->nproc != 1 || d->nprocr != 1 || d->nprocc != 1 || d->nproc_summa != 1)
+    return fail(CAPS_EINVAL,
+                "one device needs nproc = nprocr = nprocc = nproc_summa = 1 (got %d %d %d %d)",
+                d->nproc, d->nprocr, d->nprocc, d->nproc_summa);
+  if (d->lda <= 0 || d->bs <= 0 || d->nrec < 0 || d->nrec > 24)
+    return fail(CAPS_EINVAL, "bad descriptor lda=%d nrec=%d bs=%d", d->lda, d->nrec, d->bs);
+  long long div = (1LL << d->nrec) * d->bs;
+  if (d->lda % div != 0)  // matrix.cpp:4-5
+    return fail(CAPS_EINVAL, "lda=%d is not a multiple of 2^nrec * bs = %lld", d->lda, div);
+  if ((long long)d->lda * d->lda > (1LL << 34))
+    return fail(CAPS_EINVAL, "lda=%d too large", d->lda);
+  return CAPS_OK;
+}
+
+int resolve_pattern(const caps_desc &d, const char *pattern, const caps_opts *o,
+                    std::string &out, int mode, int moduli, long long budget_doubles) {
+  if (pattern) {
+    out.clear();
+    for (int l = 0; l < d.nrec; l++) {
+      char c = pattern[l];
+      if (c == '\0')
+        return fail(CAPS_EPATTERN, "pattern \"%s\" shorter than nrec=%d", pattern, d.nrec);
+      if (!strchr("bBdDhH", c))
+        return fail(CAPS_EPATTERN, "invalid pattern char '%c' in \"%s\"", c, pattern);
+      out.push_back(c);
+    }
+    return CAPS_OK;
+  }
+  if (o && (o->flags & CAPS_FLAG_REFERENCE_AUTO)) {
+    out.assign((size_t)d.nrec, 'd');  // one process: DFS at every level (multiply.cpp:43-66)
+    return CAPS_OK;
+  }
+  out = choose_pattern(d, mode, moduli, budget_doubles);
+  return CAPS_OK;
+}
+
+int resolve_mode(const caps_desc &d, const caps_opts *o, int &mode, int &moduli) {
+  mode = o ? o->mode : CAPS_MODE_FP64;
+  moduli = (o && o->moduli) ? o->moduli : OZ_DEFAULT_MODULI;
+  if (mode == CAPS_MODE_AUTO) mode = choose_mode(d);
+  if (mode != CAPS_MODE_FP64 && mode != CAPS_MODE_OZAKI_INT8)
+    return fail(CAPS_EINVAL, "unknown mode %d", mode);
+  if (mode == CAPS_MODE_OZAKI_INT8 && (moduli < 2 || moduli > OZ_MAX_MODULI))
+    return fail(CAPS_EINVAL, "moduli=%d outside 2..%d", moduli, OZ_MAX_MODULI);
+  if (mode == CAPS_MODE_FP64) moduli = 0;
+  return CAPS_OK;
+}
+
+int get_plan_locked(State &s, const double *A, const double *B, double *C, const caps_desc &d,
+                    const std::string &pat, int mode, int moduli, Plan *&out) {
+  PlanKey key{A, B, C, d, pat, mode, moduli};
+  for (auto &p : s.plans)
+    if (p->key == key) {
+      out = p.get();
+      p->last_use = ++s.clock;
+      return CAPS_OK;
+    }
+  evict_locked(s, MAX_PLANS - 1);
+  auto plan = std::make_unique<Plan>();
+  plan->key = key;
+  build_plan(*plan, A, B, C, d, pat, mode, moduli, nullptr);
+  const size_t need = plan->work_bytes;
+  if (need) {
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    if (need > freeb) {
+      evict_locked(s, 0);
+      HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+      if (need > freeb)
+        return fail(CAPS_ENOMEM, "plan needs %.1f MiB of device workspace, %.1f MiB free",
+                    need / 1048576.0, freeb / 1048576.0);
+    }
+    hipError_t e = hipMalloc(&plan->d_work, need);
+    if (e != hipSuccess)
+      return fail(CAPS_ENOMEM, "hipMalloc(%zu) for workspace: %s", need, hipGetErrorString(e));
+  }
+  build_plan(*plan, A, B, C, d, pat, mode, moduli, plan->d_work);
+  if (!plan->host_tables.empty()) {
+    hipError_t e = hipMalloc(&plan->d_tables, plan->host_tables.size());
+    if (e != hipSuccess) {
+      free_plan(*plan);
+      return fail(CAPS_ENOMEM, "hipMalloc for pointer tables: %s", hipGetErrorString(e));
+    }
+    e = hipMemcpy(plan->d_tables, plan->host_tables.data(), plan->host_tables.size(),
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      free_plan(*plan);
+      return fail(CAPS_EHIP, "uploading pointer tables: %s", hipGetErrorString(e));
+    }
+  }
+  plan->last_use = ++s.clock;
+  out = plan.get();
+  s.plans.push_back(std::move(plan));
+  return CAPS_OK;
+}
+
+int capture_graph(Plan &p) {
+  HIP_TRY(hipStreamCreateWithFlags(&p.capture_stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamBeginCapture(p.capture_stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = run_steps(p, p.capture_stream);
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(p.capture_stream, &g);
+  if (le != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return fail(CAPS_EHIP, "kernel launch during capture: %s", hipGetErrorString(le));
+  }
+  if (ee != hipSuccess) return fail(CAPS_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+  hipError_t ie = hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    p.graph = nullptr;
+    return fail(CAPS_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+  }
+  return CAPS_OK;
+}
+
+int multiply_device_locked(State &s, const double *dA, const double *dB, double *dC,
+                           const caps_desc *desc, const char *pattern, const caps_opts *opts,
+                           hipStream_t stream) {
+  int rc = check_desc(desc);
+  if (rc) return rc;
+  if (!dA || !dB || !dC) return fail(CAPS_EINVAL, "NULL matrix pointer");
+  int mode, moduli;
+  rc = resolve_mode(*desc, opts, mode, moduli);
+  if (rc) return rc;
+  std::string pat;
+  rc = resolve_pattern(*desc, pattern, opts, pat, mode, moduli, s.available_doubles);
+  if (rc) return rc;
+  Plan *plan = nullptr;
+  rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, plan);
+  if (rc) return rc;
+  const bool timed = opts && (opts->flags & CAPS_FLAG_TIMERS);
+  const bool use_graph = !timed && !(opts && (opts->flags & CAPS_FLAG_NO_GRAPH));
+  if (timed) {
+    hipError_t e = run_steps_timed(*plan, stream, s.timers_ms, s.timed);
+    if (e != hipSuccess) return fail(CAPS_EHIP, "timed launch: %s", hipGetErrorString(e));
+  } else if (use_graph && !plan->graph) {
+    rc = capture_graph(*plan);
+    if (rc) return rc;
+  }
+  if (timed) {
+  } else if (use_graph) {
+    HIP_TRY(hipGraphLaunch(plan->graph, stream));
+  } else {
+    hipError_t e = run_steps(*plan, stream);
+    if (e != hipSuccess) return fail(CAPS_EHIP, "kernel launch: %s", hipGetErrorString(e));
+  }
+  s.additions += plan->additions;
+  s.multiplications += plan->multiplications;
+  s.leaves += plan->leaves;
+  s.peak_mb = plan->work_bytes / 1048576.0;
+  s.exec_pattern = plan->exec_pattern;
+  return CAPS_OK;
+}
+
+int stage_locked(State &s, size_t elems) {
+  if (s.stage_elems >= elems && s.stage[0]) return CAPS_OK;
+  for (auto &p : s.stage)
+    if (p) {
+      hipFree(p);
+      p = nullptr;
+    }
+  s.stage_elems = 0;
+  for (auto &p : s.stage) {
+    hipError_t e = hipMalloc(&p, elems * sizeof(double));
+    if (e != hipSuccess) return fail(CAPS_ENOMEM, "staging hipMalloc: %s", hipGetErrorString(e));
+  }
+  if (!s.stage_stream) HIP_TRY(hipStreamCreateWithFlags(&s.stage_stream, hipStreamNonBlocking));
+  s.stage_elems = elems;
+  return CAPS_OK;
+}
+
+}  // namespace
+
+int device_cu_count() { return st().cus; }
+
+}  // namespace caps
+
+using namespace caps;
+
+extern "C" {
+
+int caps_init(int device) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(CAPS_ENODEV, "device %d of %d", device, n);
+  HIP_TRY(hipSetDevice(device));
+  return ensure_device_locked(s);
+}
+
+const char *caps_last_error(void) { return g_error.c_str(); }
+
+int caps_verify_desc(const caps_desc *desc) { return check_desc(desc); }
+
+long long caps_num_entries(const caps_desc *d) {
+  if (!d || d->nproc <= 0 || d->nproc_summa <= 0) return -1;
+  long long lda = d->lda;
+  return lda * lda / d->nproc / d->nproc_summa;
+}
+
+int caps_multiply_device(const double *dA, const double *dB, double *dC, const caps_desc *desc,
+                         const char *pattern, const caps_opts *opts, void *stream) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = ensure_device_locked(s);
+  if (rc) return rc;
+  return multiply_device_locked(s, dA, dB, dC, desc, pattern, opts, (hipStream_t)stream);
+}
+
+int caps_multiply(const double *A, const double *B, double *C, const caps_desc *desc,
+                  const char *pattern, const caps_opts *opts) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = ensure_device_locked(s);
+  if (rc) return rc;
+  rc = check_desc(desc);
+  if (rc) return rc;
+  if (!A || !B || !C) return fail(CAPS_EINVAL, "NULL matrix pointer");
+  const size_t n2 = (size_t)desc->lda * desc->lda;
+  rc = stage_locked(s, n2);
+  if (rc) return rc;
+  hipStream_t ss = s.stage_stream;
+  HIP_TRY(hipMemcpyAsync(s.stage[0], A, n2 * sizeof(double), hipMemcpyHostToDevice, ss));
+  HIP_TRY(hipMemcpyAsync(s.stage[1], B, n2 * sizeof(double), hipMemcpyHostToDevice, ss));
+  rc = multiply_device_locked(s, s.stage[0], s.stage[1], s.stage[2], desc, pattern, opts, ss);
+  if (rc) {
+    hipStreamSynchronize(ss);
+    return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(C, s.stage[2], n2 * sizeof(double), hipMemcpyDeviceToHost, ss));
+  HIP_TRY(hipStreamSynchronize(ss));
+  return CAPS_OK;
+}
+
+static int layout_device(const double *src, double *dst, const caps_desc *desc, void *stream,
+                         bool to_caps) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = ensure_device_locked(s);
+  if (rc) return rc;
+  rc = check_desc(desc);
+  if (rc) return rc;
+  if (!src || !dst) return fail(CAPS_EINVAL, "NULL matrix pointer");
+  if (src == dst) return fail(CAPS_EINVAL, "layout conversion cannot run in place");
+  hipError_t e = to_caps ? launch_to_caps(src, dst, desc->lda, desc->nrec, desc->bs,
+                                          (hipStream_t)stream)
+                         : launch_from_caps(src, dst, desc->lda, desc->nrec, desc->bs,
+                                            (hipStream_t)stream);
+  if (e != hipSuccess) return fail(CAPS_EHIP, "layout kernel: %s", hipGetErrorString(e));
+  return CAPS_OK;
+}
+
+int caps_to_layout_device(const double *dcm, double *dcaps, const caps_desc *desc, void *stream) {
+  return layout_device(dcm, dcaps, desc, stream, true);
+}
+
+int caps_from_layout_device(const double *dcaps, double *dcm, const caps_desc *desc,
+                            void *stream) {
+  return layout_device(dcaps, dcm, desc, stream, false);
+}
+
+static int layout_host(const double *src, double *dst, const caps_desc *desc, bool to_caps) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = ensure_device_locked(s);
+  if (rc) return rc;
+  rc = check_desc(desc);
+  if (rc) return rc;
+  if (!src || !dst) return fail(CAPS_EINVAL, "NULL matrix pointer");
+  const size_t n2 = (size_t)desc->lda * desc->lda;
+  rc = stage_locked(s, n2);
+  if (rc) return rc;
+  hipStream_t ss = s.stage_stream;
+  HIP_TRY(hipMemcpyAsync(s.stage[0], src, n2 * sizeof(double), hipMemcpyHostToDevice, ss));
+  hipError_t e = to_caps ? launch_to_caps(s.stage[0], s.stage[1], desc->lda, desc->nrec,
+                                          desc->bs, ss)
+                         : launch_from_caps(s.stage[0], s.stage[1], desc->lda, desc->nrec,
+                                            desc->bs, ss);
+  if (e != hipSuccess) return fail(CAPS_EHIP, "layout kernel: %s", hipGetErrorString(e));
+  HIP_TRY(hipMemcpyAsync(dst, s.stage[1], n2 * sizeof(double), hipMemcpyDeviceToHost, ss));
+  HIP_TRY(hipStreamSynchronize(ss));
+  return CAPS_OK;
+}
+
+int caps_to_layout(const double *cm, double *caps, const caps_desc *desc) {
+  return layout_host(cm, caps, desc, true);
+}
+
+int caps_from_layout(const double *caps, double *cm, const caps_desc *desc) {
+  return layout_host(caps, cm, desc, false);
+}
+
+int caps_get_counters(caps_counters *out) {
+  if (!out) return fail(CAPS_EINVAL, "NULL counters");
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  memset(out, 0, sizeof(*out));
+  out->additions = s.additions;
+  out->multiplications = s.multiplications;
+  out->leaf_products = s.leaves;
+  out->peak_workspace_mb = s.peak_mb;
+  snprintf(out->pattern, sizeof(out->pattern), "%s", s.exec_pattern.c_str());
+  for (int k = 0; k < CAPS_NUM_TIMERS; k++) {
+    out->timers_ms[k] = s.timers_ms[k];
+    out->timed_launches[k] = s.timed[k];
+  }
+  return CAPS_OK;
+}
+
+int caps_reset_counters(void) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  s.additions = s.multiplications = s.leaves = 0;
+  s.peak_mb = 0;
+  s.exec_pattern.clear();
+  for (int k = 0; k < CAPS_NUM_TIMERS; k++) {
+    s.timers_ms[k] = 0;
+    s.timed[k] = 0;
+  }
+  return CAPS_OK;
+}
+
+int caps_set_available_memory(long long doubles) {
+  if (doubles <= 0) return fail(CAPS_EINVAL, "available memory must be positive");
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  s.available_doubles = doubles;
+  return CAPS_OK;
+}
+
+int caps_release(void) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  evict_locked(s, 0);
+  for (auto &p : s.stage)
+    if (p) {
+      hipFree(p);
+      p = nullptr;
+    }
+  s.stage_elems = 0;
+  return CAPS_OK;
+}
+
+}  // extern "C"

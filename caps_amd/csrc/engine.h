@@ -1,0 +1,86 @@
+// Host engine: turns (descriptor, pattern, mode) into a list of batched
+// device launches, owns the device workspace and replays the list as a
+// captured hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/caps_amd.h"
+#include "kernels.h"
+
+namespace caps {
+
+// one sub-problem of a batched level: C = A * B, all s x s in CAPS layout
+struct Prob {
+  const double *A;
+  const double *B;
+  double *C;
+};
+
+enum StepKind : int {
+  STEP_EW = 0,       // fused Winograd adds (EwOp in `op`)
+  STEP_GEMM_F64 = 1, // batched FP64 leaf GEMM
+  STEP_REORDER = 2,  // bs-block <-> column-major leaf (op = 1: to column-major)
+  STEP_OZ_SCALE = 3, // split mode: per-row / per-column scaling exponents
+  STEP_OZ_SPLIT = 4, // split mode: scaled operands -> INT8 residue planes
+  STEP_OZ_GEMM = 5,  // split mode: INT8 GEMM per modulus, residue outputs
+  STEP_OZ_RECON = 6, // split mode: CRT reconstruction + unscaling
+};
+
+struct Step {
+  int kind;
+  int op;
+  int count;      // sub-problems in the batch
+  long long n;    // elements per quadrant (EW) or leaf size m (GEMM/REORDER)
+  int bs;
+  int vec2;
+  size_t table;   // byte offset of this step's pointer table
+  size_t aux;     // byte offset of an auxiliary workspace region
+};
+
+struct PlanKey {
+  const void *A, *B, *C;
+  caps_desc desc;
+  std::string pattern;
+  int mode, moduli;
+  bool operator==(const PlanKey &o) const;
+};
+
+struct Plan {
+  PlanKey key;
+  std::vector<Step> steps;
+  std::vector<unsigned char> host_tables;
+  void *d_tables = nullptr;
+  void *d_work = nullptr;
+  size_t work_bytes = 0;
+  hipGraphExec_t graph = nullptr;
+  hipStream_t capture_stream = nullptr;
+  long long additions = 0, multiplications = 0, leaves = 0;
+  std::string exec_pattern;
+  unsigned long long last_use = 0;
+  // split-mode constants (moduli, CRT weights), device copy in d_tables
+  size_t oz_const = 0;
+  int oz_moduli = 0;
+};
+
+// Build a plan: pattern must already be resolved to one char per level.
+// work_base = nullptr computes sizes only.
+int build_plan(Plan &plan, const double *A, const double *B, double *C,
+               const caps_desc &d, const std::string &pattern, int mode, int moduli,
+               void *work_base);
+
+// Enqueue every step of the plan on `s` (no graph).
+hipError_t run_steps(const Plan &plan, hipStream_t s);
+
+// Same, with a HIP event pair around every step; adds each step's elapsed
+// milliseconds to ms[timer of its kind] and counts launches in n[].  Waits
+// for the stream.
+hipError_t run_steps_timed(const Plan &plan, hipStream_t s, double *ms, long long *n);
+
+// Workspace (bytes) a resolved pattern needs.
+size_t plan_workspace_bytes(const caps_desc &d, const std::string &pattern, int mode,
+                            int moduli);
+
+}  // namespace caps

@@ -1,0 +1,166 @@
+// Batched FP64 leaf GEMM on the gfx950 matrix cores (replaces the BLAS call
+// square_dgemm_zero, dgemm-blas.cpp:4-13: C = A*B, beta = 0, column-major).
+//
+// MFMA-bound: v_mfma_f64_16x16x4_f64 (2048 flop per instruction).
+// Block tile 128 x 128, 4 waves in a 2 x 2 grid, each wave 64 x 64 = 4 x 4
+// MFMA tiles (64 accumulator doubles per lane).  K is staged through LDS in
+// slices of 16 with double buffering: the next slice's global loads are in
+// flight while the current slice feeds the MFMAs, one barrier per slice.
+//
+// LDS: A slice stored [k][row] with row stride 144 doubles and B slice stored
+// [col][k] with col stride 18 doubles; with these strides the 64 lanes of an
+// MFMA operand read touch every bank exactly twice (two passes of 256 bytes).
+//
+// One persistent launch covers all tiles of all problems of the batch (the
+// 7^b sub-products of the batched levels).  Consecutive tile ids stay on the
+// same XCD (blocks are dispatched round-robin over the 8 XCDs) and sweep an
+// 8 x 4 patch of tiles, so one XCD's L2 sees 8 A row panels and 4 B column
+// panels of one problem at a time.
+#include "kernels.h"
+#include "tilemap.h"
+
+namespace caps {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+constexpr int BM = 128, BN = 128, BK = 16;
+constexpr int SA = BM + 16;  // As[k][row]
+constexpr int SB = BK + 2;   // Bs[col][k]
+constexpr int NT = 256;
+// global -> registers: the A slice (128 rows x 16 k) and B slice (16 k x 128
+// cols) of one K step, two doubles per access
+template <bool CHECK>
+__device__ __forceinline__ void gload_f64(const double *__restrict__ A,
+                                          const double *__restrict__ B, int m, int row0,
+                                          int col0, int k0, int tid, v2d (&ra)[4],
+                                          v2d (&rb)[4]) {
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    const int idx = tid + NT * it;
+    // A: column kc of the slice, rows 2*r2, 2*r2+1
+    const int kc = idx >> 6, r2 = idx & 63;
+    const int ar = row0 + 2 * r2, ak = k0 + kc;
+    // B: column c of the tile, k rows 2*k2, 2*k2+1
+    const int c = idx >> 3, k2 = idx & 7;
+    const int bk = k0 + 2 * k2, bc = col0 + c;
+    if (!CHECK) {
+      ra[it] = *reinterpret_cast<const v2d *>(A + (long long)ak * m + ar);
+      rb[it] = *reinterpret_cast<const v2d *>(B + (long long)bc * m + bk);
+    } else {
+      const bool kok = ak < m;
+      ra[it].x = (kok && ar < m) ? A[(long long)ak * m + ar] : 0.0;
+      ra[it].y = (kok && ar + 1 < m) ? A[(long long)ak * m + ar + 1] : 0.0;
+      const bool cok = bc < m;
+      rb[it].x = (cok && bk < m) ? B[(long long)bc * m + bk] : 0.0;
+      rb[it].y = (cok && bk + 1 < m) ? B[(long long)bc * m + bk + 1] : 0.0;
+    }
+  }
+}
+
+// registers -> LDS: As[k][row], Bs[col][k]
+__device__ __forceinline__ void lstore_f64(double *As, double *Bs, int tid,
+                                           const v2d (&ra)[4], const v2d (&rb)[4]) {
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    const int idx = tid + NT * it;
+    const int kc = idx >> 6, r2 = idx & 63;
+    *reinterpret_cast<v2d *>(&As[kc * SA + 2 * r2]) = ra[it];
+    const int c = idx >> 3, k2 = idx & 7;
+    *reinterpret_cast<v2d *>(&Bs[c * SB + 2 * k2]) = rb[it];
+  }
+}
+
+template <bool CHECK>
+__global__ __launch_bounds__(NT) void dgemm_kernel(const GemmPtrs *__restrict__ tab, int m,
+                                                   TileMap map) {
+  __shared__ double As[2][BK * SA];
+  __shared__ double Bs[2][BN * SB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l16 = lane & 15, lk = lane >> 4;
+  const int G = gridDim.x;
+  const int nk = (m + BK - 1) / BK;
+
+  for (int round = 0; round * G < map.total; round++) {
+    const int tile = round * G + xcd_tile(blockIdx.x, G);
+    if (tile >= map.total) break;
+    int prob, ti, tj;
+    tile_coords(map, tile, prob, ti, tj);
+    const double *__restrict__ A = tab[prob].A;
+    const double *__restrict__ B = tab[prob].B;
+    double *__restrict__ C = tab[prob].C;
+    const int row0 = ti * BM, col0 = tj * BN;
+
+    v4d acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[r][c] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    v2d ra[4], rb[4];
+    gload_f64<CHECK>(A, B, m, row0, col0, 0, tid, ra, rb);
+    lstore_f64(As[0], Bs[0], tid, ra, rb);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+      const int buf = kt & 1;
+      if (kt + 1 < nk) gload_f64<CHECK>(A, B, m, row0, col0, (kt + 1) * BK, tid, ra, rb);
+#pragma unroll
+      for (int kk = 0; kk < BK / 4; kk++) {
+        const int ka = kk * 4 + lk;
+        double a[4], b[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) a[r] = As[buf][ka * SA + wm * 64 + r * 16 + l16];
+#pragma unroll
+        for (int c = 0; c < 4; c++) b[c] = Bs[buf][(wn * 64 + c * 16 + l16) * SB + ka];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+      }
+      if (kt + 1 < nk) lstore_f64(As[buf ^ 1], Bs[buf ^ 1], tid, ra, rb);
+      __syncthreads();
+    }
+
+    // D layout of v_mfma_f64_16x16x4 on gfx950: lane holds column (lane % 16)
+    // and rows (lane / 16) + 4 v in its four result registers v = 0..3
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int row = row0 + wm * 64 + r * 16 + lk;
+        const int col = col0 + wn * 64 + c * 16 + l16;
+        double *dst = C + (long long)col * m + row;
+        if (!CHECK) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) dst[4 * v] = acc[r][c][v];
+        } else if (col < m) {
+#pragma unroll
+          for (int v = 0; v < 4; v++)
+            if (row + 4 * v < m) dst[4 * v] = acc[r][c][v];
+        }
+      }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_dgemm_batched(const GemmPtrs *tab, int count, int m, hipStream_t s) {
+  if (count <= 0 || m <= 0) return hipSuccess;
+  bool ok;
+  TileMap map = make_tilemap(m, BM, count, ok);
+  if (!ok) return hipErrorInvalidValue;
+  const bool fast = (m % BM == 0);
+  // persistent grid: 2 blocks per CU, a multiple of the XCD count
+  const int G = persistent_grid(2, device_cu_count(), map.total);
+  if (fast)
+    hipLaunchKernelGGL((dgemm_kernel<false>), dim3(G), dim3(NT), 0, s, tab, m, map);
+  else
+    hipLaunchKernelGGL((dgemm_kernel<true>), dim3(G), dim3(NT), 0, s, tab, m, map);
+  return hipGetLastError();
+}
+
+}  // namespace caps

@@ -1,0 +1,71 @@
+// Split-precision leaf product (Ozaki scheme II: integer modular emulation of
+// an FP64 GEMM; K. Ozaki, Y. Uchino, T. Imamura, "Ozaki Scheme II: A
+// GEMM-oriented emulation of floating-point matrix multiplication using an
+// integer modular technique", 2025).  No counterpart in the reference, whose
+// leaf is a BLAS dgemm (dgemm-blas.cpp:4-13).
+//
+// For a leaf product C = A * B (m x m, column-major FP64):
+//  1. scale: row i of A by 2^sa_i, column j of B by 2^sb_j, powers of two
+//     chosen from deterministic 2-norm upper bounds so that
+//        sum_k |A'_ik| |B'_kj| <= ||A'_i.|| ||B'_.j|| <= L^2 < M / 2,
+//     with A' = trunc(2^sa A), B' = trunc(2^sb B) (exact FP64 integers) and
+//     M the product of the moduli;
+//  2. split: A' mod m_l and B' mod m_l as symmetric residues (INT8 planes),
+//     A-side planes row-major (k contiguous), B-side column-major;
+//  3. product: C'_l = A'_l B'_l on the INT8 matrix cores with exact INT32
+//     accumulation, reduced mod m_l in the epilogue (INT8 output plane);
+//  4. reconstruct: X = sum_l C'_l W_l  (mod M) by the Chinese remainder
+//     theorem in FP64 with the weights W_l split into 37-bit pieces, so every
+//     partial sum is exact; X = A'B' exactly, returned correctly rounded to
+//     FP64, then C = 2^-(sa_i + sb_j) X.
+// Integer-valued leaves whose products fit 53 bits come out exact.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace caps {
+
+constexpr int OZ_MAX_MODULI = 20;
+constexpr int OZ_MAX_PIECES = 5;
+constexpr int OZ_PIECE_BITS = 37;
+constexpr int OZ_DEFAULT_MODULI = 14;
+// row / column chunks of the deterministic sum-of-squares reduction
+constexpr int OZ_CHUNKS = 8;
+
+struct OzakiConsts {
+  int nmod;
+  int npieces;
+  int m[OZ_MAX_MODULI];
+  double inv_m[OZ_MAX_MODULI];
+  double w[OZ_MAX_MODULI][OZ_MAX_PIECES];  // CRT weight W_l in base 2^37 digits
+  double Mp[OZ_MAX_PIECES];                // M in base 2^37 digits
+  double invM;                             // 1 / M (rounded)
+  double L;                                // per-side norm bound, L^2 < M/2
+  double log2M;
+};
+
+struct OzLeaf {
+  const double *A;  // m x m column-major
+  const double *B;
+  double *C;
+  int *ea;          // m row exponents of A
+  int *eb;          // m column exponents of B
+  int8_t *ra;       // nmod planes, m x m row-major
+  int8_t *rb;       // nmod planes, m x m column-major
+  int8_t *rc;       // nmod planes, m x m column-major (product residues)
+  double *pa;       // OZ_CHUNKS x m partial row sums of squares (A)
+  double *pb;       // m column sums of squares (B)
+};
+
+// Host: moduli and CRT constants for nmod moduli (2..OZ_MAX_MODULI).
+// Returns false if nmod is out of range.
+bool ozaki_make_consts(int nmod, OzakiConsts &k);
+
+// The moduli used (greedy pairwise-coprime, descending from 256).
+const int *ozaki_moduli_table();
+
+// step: 0 = scale, 1 = split, 2 = INT8 GEMM, 3 = reconstruct
+hipError_t launch_ozaki_step(int step, const OzLeaf *d_tab, int count, int m, int nmod,
+                             const OzakiConsts *d_consts, hipStream_t s);
+
+}  // namespace caps

@@ -1,0 +1,521 @@
+// Split-precision leaf kernels (Ozaki scheme II over INT8 moduli); the
+// scheme is described in ozaki.h.
+//
+//   scale  : deterministic 2-norm upper bounds of the rows of A / columns of
+//            B as (exponent, scaled sum of squares) pairs reduced in a fixed
+//            order, then the power-of-two scale exponents.      HBM-bound
+//   split  : A' = trunc(2^sa A), residues mod m_l -> INT8 planes (A planes
+//            transposed to row-major through LDS).                HBM-bound
+//   igemm  : INT8 planes -> INT32 products on v_mfma_i32_16x16x64_i8, mod m_l
+//            in the epilogue, INT8 residue planes out.            MFMA-bound
+//   recon  : CRT in FP64 from the residue planes, unscale.        HBM-bound
+#include <climits>
+
+#include "kernels.h"
+#include "ozaki.h"
+#include "tilemap.h"
+
+namespace caps {
+
+namespace {
+
+constexpr int NOSCALE = -100000;  // exponent of an all-zero row / column
+
+// running (exponent e, sum s) with sum of squares = s * 4^e
+__device__ __forceinline__ void sq_acc(double a, int &e, double &s) {
+  if (a == 0.0) return;
+  int ea = ilogb(a);
+  if (ea > e) {
+    s = (e == NOSCALE) ? 0.0 : ldexp(s, 2 * (e - ea));
+    e = ea;
+  }
+  double t = ldexp(a, -e);
+  s = fma(t, t, s);
+}
+
+__device__ __forceinline__ void sq_merge(int &e, double &s, int e2, double s2) {
+  if (e2 == NOSCALE) return;
+  if (e == NOSCALE) {
+    e = e2;
+    s = s2;
+    return;
+  }
+  if (e2 > e) {
+    s = ldexp(s, 2 * (e - e2)) + s2;
+    e = e2;
+  } else {
+    s = s + ldexp(s2, 2 * (e2 - e));
+  }
+}
+
+// largest integer t with 2^t * 2^e * sqrt(s) * (1 + 2^-20) <= L
+__device__ __forceinline__ int scale_exponent(int e, double s, double L) {
+  if (e == NOSCALE || s == 0.0) return 0;
+  const double nm = sqrt(s) * (1.0 + 0x1p-20);  // rounding of s covered for K < 2^30
+  int t = ilogb(L) - e - ilogb(nm);
+  if (ldexp(nm, e + t) > L) t -= 1;
+  return t;
+}
+
+// symmetric residue of the FP64 integer x (|x| < 2^60) modulo mi
+__device__ __forceinline__ int sym_residue(double x, int mi, double inv) {
+  const double q = rint(x * inv);
+  const double r = fma(-q, (double)mi, x);  // exact: |r| < m
+  int ri = (int)r;
+  const int hi = (mi - 1) >> 1, lo = -(mi >> 1);
+  ri = ri > hi ? ri - mi : ri;
+  ri = ri < lo ? ri + mi : ri;
+  return ri;
+}
+
+constexpr int T256 = 256;
+
+// ------------------------------------------------------------------ scale
+// A: thread per (row, chunk of columns), sequential over the chunk
+__global__ __launch_bounds__(T256) void oz_rowsq_kernel(const OzLeaf *__restrict__ tab, int m) {
+  const OzLeaf &L = tab[blockIdx.z];
+  const int i = blockIdx.x * T256 + threadIdx.x;
+  if (i >= m) return;
+  const int c = blockIdx.y;
+  const int per = (m + OZ_CHUNKS - 1) / OZ_CHUNKS;
+  const int k0 = c * per, k1 = min(m, k0 + per);
+  int e = NOSCALE;
+  double s = 0.0;
+  for (int k = k0; k < k1; k++) sq_acc(L.A[(long long)k * m + i], e, s);
+  reinterpret_cast<double2 *>(L.pa)[(long long)c * m + i] = make_double2(s, (double)e);
+}
+
+// B: one wave per column, lanes strided, fixed butterfly, lane 0 keeps
+__global__ __launch_bounds__(T256) void oz_colsq_kernel(const OzLeaf *__restrict__ tab, int m) {
+  const OzLeaf &L = tab[blockIdx.y];
+  const int j = blockIdx.x * (T256 / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= m) return;
+  const double *col = L.B + (long long)j * m;
+  int e = NOSCALE;
+  double s = 0.0;
+  for (int k = lane; k < m; k += 64) sq_acc(col[k], e, s);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    int e2 = __shfl_down(e, off, 64);
+    double s2 = __shfl_down(s, off, 64);
+    if (lane < off) sq_merge(e, s, e2, s2);
+  }
+  if (lane == 0) reinterpret_cast<double2 *>(L.pb)[j] = make_double2(s, (double)e);
+}
+
+// exponents: ea from the OZ_CHUNKS partials in chunk order, eb from pb
+__global__ __launch_bounds__(T256) void oz_finalize_kernel(const OzLeaf *__restrict__ tab, int m,
+                                                           const OzakiConsts *__restrict__ kc) {
+  const OzLeaf &L = tab[blockIdx.y];
+  const int i = blockIdx.x * T256 + threadIdx.x;
+  if (i >= m) return;
+  const double2 *pa = reinterpret_cast<const double2 *>(L.pa);
+  int e = NOSCALE;
+  double s = 0.0;
+  for (int c = 0; c < OZ_CHUNKS; c++) {
+    double2 v = pa[(long long)c * m + i];
+    sq_merge(e, s, (int)v.y, v.x);
+  }
+  L.ea[i] = scale_exponent(e, s, kc->L);
+  double2 v = reinterpret_cast<const double2 *>(L.pb)[i];
+  L.eb[i] = scale_exponent((int)v.y, v.x, kc->L);
+}
+
+// ------------------------------------------------------------------ split
+// A (column-major) -> nmod row-major INT8 planes; 64 x 64 tiles through LDS.
+template <bool CHECK>
+__global__ __launch_bounds__(T256) void oz_split_a_kernel(const OzLeaf *__restrict__ tab, int m,
+                                                          int nmod,
+                                                          const OzakiConsts *__restrict__ kc) {
+  __shared__ double T[64][65];  // [col][row]
+  const OzLeaf &L = tab[blockIdx.y];
+  const int tiles = (m + 63) / 64;
+  const int ti = blockIdx.x % tiles, tk = blockIdx.x / tiles;
+  const int r0 = ti * 64, k0 = tk * 64;
+  // load: lane -> row, loop over columns (coalesced 512 B per column)
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += T256) {
+    const int r = idx & 63, c = idx >> 6;
+    const int gr = r0 + r, gc = k0 + c;
+    T[c][r] = (!CHECK || (gr < m && gc < m)) ? L.A[(long long)gc * m + gr] : 0.0;
+  }
+  __syncthreads();
+  // each thread: one row, 16 consecutive k
+  const int r = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
+  const int gr = r0 + r;
+  if (CHECK && gr >= m) return;
+  const int sa = L.ea[gr];
+  double x[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) x[c] = trunc(ldexp(T[kq + c][r], sa));
+  int8_t *base = L.ra + (long long)gr * m + k0 + kq;
+  const long long plane = (long long)m * m;
+  for (int l = 0; l < nmod; l++) {
+    const int mi = kc->m[l];
+    const double inv = kc->inv_m[l];
+    if (!CHECK) {
+      unsigned w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        unsigned v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+          v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv) & 0xff) << (8 * b);
+        w[q] = v;
+      }
+      *reinterpret_cast<uint4 *>(base + l * plane) = make_uint4(w[0], w[1], w[2], w[3]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+        if (k0 + kq + c < m) base[l * plane + c] = (int8_t)sym_residue(x[c], mi, inv);
+    }
+  }
+}
+
+// B (column-major) -> nmod column-major INT8 planes; thread per 16 rows of a column
+template <bool CHECK>
+__global__ __launch_bounds__(T256) void oz_split_b_kernel(const OzLeaf *__restrict__ tab, int m,
+                                                          int nmod,
+                                                          const OzakiConsts *__restrict__ kc) {
+  const OzLeaf &L = tab[blockIdx.y];
+  const long long total16 = ((long long)m * m + 15) / 16;
+  const long long plane = (long long)m * m;
+  for (long long u = (long long)blockIdx.x * T256 + threadIdx.x; u < total16;
+       u += (long long)gridDim.x * T256) {
+    const long long e0 = u * 16;
+    double x[16];
+    int sb[16];
+    if (!CHECK) {
+      // m % 16 == 0: the 16 elements share a column
+      const int j = (int)(e0 / m);
+      const int s = L.eb[j];
+#pragma unroll
+      for (int c = 0; c < 16; c += 2) {
+        double2 v = *reinterpret_cast<const double2 *>(L.B + e0 + c);
+        x[c] = trunc(ldexp(v.x, s));
+        x[c + 1] = trunc(ldexp(v.y, s));
+      }
+      for (int l = 0; l < nmod; l++) {
+        const int mi = kc->m[l];
+        const double inv = kc->inv_m[l];
+        unsigned w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          unsigned v = 0;
+#pragma unroll
+          for (int b = 0; b < 4; b++)
+            v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv) & 0xff) << (8 * b);
+          w[q] = v;
+        }
+        *reinterpret_cast<uint4 *>(L.rb + l * plane + e0) = make_uint4(w[0], w[1], w[2], w[3]);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 16; c++) {
+        const long long e = e0 + c;
+        sb[c] = e < plane ? L.eb[(int)(e / m)] : 0;
+        x[c] = e < plane ? trunc(ldexp(L.B[e], sb[c])) : 0.0;
+      }
+      for (int l = 0; l < nmod; l++) {
+        const int mi = kc->m[l];
+        const double inv = kc->inv_m[l];
+#pragma unroll
+        for (int c = 0; c < 16; c++)
+          if (e0 + c < plane) L.rb[l * plane + e0 + c] = (int8_t)sym_residue(x[c], mi, inv);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ INT8 GEMM
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+constexpr int IBM = 128, IBN = 128, IBK = 64;
+constexpr int ISTR = IBK + 16;  // LDS row stride (bytes): b128 reads of 16 lanes hit 64 banks
+
+// symmetric residue of an INT32 product sum (|x| < 2^31) modulo mi
+__device__ __forceinline__ int mod_i32(int x, int mi, float inv) {
+  int q = __float2int_rn((float)x * inv);
+  int r = x - q * mi;
+  const int hi = (mi - 1) >> 1, lo = -(mi >> 1);
+  r = r > hi ? r - mi : r;
+  r = r < lo ? r + mi : r;
+  return r;
+}
+
+// global -> registers: 128 rows of A (k contiguous) and 128 columns of B
+// (k contiguous), 64 k each, 16 bytes per access
+template <bool CHECK>
+__device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ap,
+                                         const int8_t *__restrict__ Bp, int m, int row0,
+                                         int col0, int k0, int tid, v4u (&ga)[2],
+                                         v4u (&gb)[2]) {
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const int idx = tid + T256 * it;
+    const int r = idx >> 2, kq = (idx & 3) * 16;
+    if (!CHECK) {
+      ga[it] = *reinterpret_cast<const v4u *>(Ap + (long long)(row0 + r) * m + k0 + kq);
+      gb[it] = *reinterpret_cast<const v4u *>(Bp + (long long)(col0 + r) * m + k0 + kq);
+    } else {
+      unsigned wa[4], wb[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        unsigned va = 0, vb = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int k = k0 + kq + 4 * q + b;
+          if (k < m && row0 + r < m)
+            va |= (unsigned)(uint8_t)Ap[(long long)(row0 + r) * m + k] << (8 * b);
+          if (k < m && col0 + r < m)
+            vb |= (unsigned)(uint8_t)Bp[(long long)(col0 + r) * m + k] << (8 * b);
+        }
+        wa[q] = va;
+        wb[q] = vb;
+      }
+      ga[it] = v4u{wa[0], wa[1], wa[2], wa[3]};
+      gb[it] = v4u{wb[0], wb[1], wb[2], wb[3]};
+    }
+  }
+}
+
+__device__ __forceinline__ void lstore_i8(int8_t *As, int8_t *Bs, int tid, const v4u (&ga)[2],
+                                          const v4u (&gb)[2]) {
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const int idx = tid + T256 * it;
+    const int r = idx >> 2, kq = (idx & 3) * 16;
+    *reinterpret_cast<v4u *>(&As[r * ISTR + kq]) = ga[it];
+    *reinterpret_cast<v4u *>(&Bs[r * ISTR + kq]) = gb[it];
+  }
+}
+
+// problem p = leaf * nmod + l: C_l = A_l * B_l (mod m_l)
+template <bool CHECK>
+__global__ __launch_bounds__(T256) void oz_igemm_kernel(const OzLeaf *__restrict__ tab, int m,
+                                                        int nmod,
+                                                        const OzakiConsts *__restrict__ kc,
+                                                        TileMap map) {
+  __shared__ __attribute__((aligned(16))) int8_t As[2][IBM * ISTR];
+  __shared__ __attribute__((aligned(16))) int8_t Bs[2][IBN * ISTR];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l16 = lane & 15, lk = lane >> 4;
+  const int G = gridDim.x;
+  const int nk = (m + IBK - 1) / IBK;
+  const long long plane = (long long)m * m;
+
+  for (int round = 0; round * G < map.total; round++) {
+    const int tile = round * G + xcd_tile(blockIdx.x, G);
+    if (tile >= map.total) break;
+    int prob, ti, tj;
+    tile_coords(map, tile, prob, ti, tj);
+    const int leaf = prob / nmod, l = prob - leaf * nmod;
+    const int8_t *__restrict__ Ap = tab[leaf].ra + l * plane;  // row-major
+    const int8_t *__restrict__ Bp = tab[leaf].rb + l * plane;  // column-major
+    int8_t *__restrict__ Cp = tab[leaf].rc + l * plane;
+    const int mi = kc->m[l];
+    const float invf = 1.0f / (float)mi;
+    const int row0 = ti * IBM, col0 = tj * IBN;
+
+    v4i acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[r][c] = v4i{0, 0, 0, 0};
+
+    v4u ga[2], gb[2];
+    gload_i8<CHECK>(Ap, Bp, m, row0, col0, 0, tid, ga, gb);
+    lstore_i8(As[0], Bs[0], tid, ga, gb);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+      const int buf = kt & 1;
+      if (kt + 1 < nk) gload_i8<CHECK>(Ap, Bp, m, row0, col0, (kt + 1) * IBK, tid, ga, gb);
+      v4i a[4], b[4];
+      // lane group lk supplies k = 16*lk .. 16*lk+15 for both operands
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        a[r] = *reinterpret_cast<const v4i *>(&As[buf][(wm * 64 + r * 16 + l16) * ISTR + 16 * lk]);
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        b[c] = *reinterpret_cast<const v4i *>(&Bs[buf][(wn * 64 + c * 16 + l16) * ISTR + 16 * lk]);
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], b[c], acc[r][c], 0, 0, 0);
+      if (kt + 1 < nk) lstore_i8(As[buf ^ 1], Bs[buf ^ 1], tid, ga, gb);
+      __syncthreads();
+    }
+
+    // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int row = row0 + wm * 64 + r * 16 + 4 * lk;
+        const int col = col0 + wn * 64 + c * 16 + l16;
+        int8_t *dst = Cp + (long long)col * m + row;
+        if (!CHECK) {
+          unsigned v = 0;
+#pragma unroll
+          for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
+          *reinterpret_cast<unsigned *>(dst) = v;
+        } else if (col < m) {
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (row + q < m) dst[q] = (int8_t)mod_i32(acc[r][c][q], mi, invf);
+        }
+      }
+  }
+}
+
+// ------------------------------------------------------------------ reconstruct
+// S_p += r_l * W_l[p]: exact, every partial sum < 2^49
+template <int NP>
+__device__ __forceinline__ void crt_acc(double (&S)[NP], int r, int l,
+                                        const OzakiConsts *__restrict__ kc) {
+  const double rd = (double)r;
+#pragma unroll
+  for (int p = 0; p < NP; p++) S[p] = fma(rd, kc->w[l][p], S[p]);
+}
+
+// X = sum_p S_p 2^(37p) reduced to (-M/2, M/2), correctly rounded
+template <int NP>
+__device__ __forceinline__ double crt_finish(const double (&S)[NP],
+                                             const OzakiConsts *__restrict__ kc) {
+  double approx = S[NP - 1];
+#pragma unroll
+  for (int p = NP - 2; p >= 0; p--) approx = fma(approx, 0x1p37, S[p]);
+  const double k = rint(approx * kc->invM);
+  double X = fma(-k, kc->Mp[NP - 1], S[NP - 1]);  // exact
+#pragma unroll
+  for (int p = NP - 2; p >= 0; p--) X = fma(X, 0x1p37, fma(-k, kc->Mp[p], S[p]));
+  return X;
+}
+
+template <int NP, bool VEC>
+__global__ __launch_bounds__(T256) void oz_recon_kernel(const OzLeaf *__restrict__ tab, int m,
+                                                        int nmod,
+                                                        const OzakiConsts *__restrict__ kc) {
+  const OzLeaf &L = tab[blockIdx.y];
+  const long long plane = (long long)m * m;
+  const int per = VEC ? 4 : 1;
+  const long long units = plane / per;
+  for (long long u = (long long)blockIdx.x * T256 + threadIdx.x; u < units;
+       u += (long long)gridDim.x * T256) {
+    const long long e0 = u * per;
+    const int j = (int)(e0 / m), i0 = (int)(e0 - (long long)j * m);
+    const int sb = L.eb[j];
+    if (VEC) {
+      double S[4][NP];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int p = 0; p < NP; p++) S[q][p] = 0.0;
+      for (int l = 0; l < nmod; l++) {
+        const unsigned w = *reinterpret_cast<const unsigned *>(L.rc + l * plane + e0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) crt_acc<NP>(S[q], (int)(int8_t)((w >> (8 * q)) & 0xff), l, kc);
+      }
+      double out[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) out[q] = ldexp(crt_finish<NP>(S[q], kc), -(L.ea[i0 + q] + sb));
+      reinterpret_cast<double2 *>(L.C + e0)[0] = make_double2(out[0], out[1]);
+      reinterpret_cast<double2 *>(L.C + e0)[1] = make_double2(out[2], out[3]);
+    } else {
+      double S[NP];
+#pragma unroll
+      for (int p = 0; p < NP; p++) S[p] = 0.0;
+      for (int l = 0; l < nmod; l++) crt_acc<NP>(S, (int)L.rc[l * plane + e0], l, kc);
+      L.C[e0] = ldexp(crt_finish<NP>(S, kc), -(L.ea[i0] + sb));
+    }
+  }
+}
+
+template <int NP>
+hipError_t launch_recon(const OzLeaf *tab, int count, int m, int nmod, const OzakiConsts *kc,
+                        hipStream_t s) {
+  const bool vec = (m % 4 == 0);
+  const long long units = (long long)m * m / (vec ? 4 : 1);
+  long long blocks = (units + T256 - 1) / T256;
+  long long cap = (long long)device_cu_count() * 8 / count;
+  if (cap < 1) cap = 1;
+  if (blocks > cap) blocks = cap;
+  dim3 grid((unsigned)blocks, (unsigned)count);
+  if (vec)
+    hipLaunchKernelGGL((oz_recon_kernel<NP, true>), grid, dim3(T256), 0, s, tab, m, nmod, kc);
+  else
+    hipLaunchKernelGGL((oz_recon_kernel<NP, false>), grid, dim3(T256), 0, s, tab, m, nmod, kc);
+  return hipGetLastError();
+}
+
+int npieces_for(int nmod) {
+  // log2 M <= 8 * nmod; pieces of 37 bits
+  return (8 * nmod + OZ_PIECE_BITS - 1) / OZ_PIECE_BITS;
+}
+
+}  // namespace
+
+hipError_t launch_ozaki_step(int step, const OzLeaf *tab, int count, int m, int nmod,
+                             const OzakiConsts *kc, hipStream_t s) {
+  if (count <= 0 || m <= 0) return hipSuccess;
+  if (count > 65535 || nmod < 2 || nmod > OZ_MAX_MODULI) return hipErrorInvalidValue;
+  switch (step) {
+    case 0: {
+      dim3 g1((m + T256 - 1) / T256, OZ_CHUNKS, count);
+      hipLaunchKernelGGL(oz_rowsq_kernel, g1, dim3(T256), 0, s, tab, m);
+      dim3 g2((m + 3) / 4, count);
+      hipLaunchKernelGGL(oz_colsq_kernel, g2, dim3(T256), 0, s, tab, m);
+      dim3 g3((m + T256 - 1) / T256, count);
+      hipLaunchKernelGGL(oz_finalize_kernel, g3, dim3(T256), 0, s, tab, m, kc);
+      return hipGetLastError();
+    }
+    case 1: {
+      const int tiles = (m + 63) / 64;
+      dim3 ga((unsigned)(tiles * tiles), count);
+      const bool fa = (m % 64 == 0);
+      if (fa)
+        hipLaunchKernelGGL((oz_split_a_kernel<false>), ga, dim3(T256), 0, s, tab, m, nmod, kc);
+      else
+        hipLaunchKernelGGL((oz_split_a_kernel<true>), ga, dim3(T256), 0, s, tab, m, nmod, kc);
+      const long long u16 = ((long long)m * m + 15) / 16;
+      long long blocks = (u16 + T256 - 1) / T256;
+      long long cap = (long long)device_cu_count() * 8 / count;
+      if (cap < 1) cap = 1;
+      if (blocks > cap) blocks = cap;
+      dim3 gb((unsigned)blocks, count);
+      if (m % 16 == 0)
+        hipLaunchKernelGGL((oz_split_b_kernel<false>), gb, dim3(T256), 0, s, tab, m, nmod, kc);
+      else
+        hipLaunchKernelGGL((oz_split_b_kernel<true>), gb, dim3(T256), 0, s, tab, m, nmod, kc);
+      return hipGetLastError();
+    }
+    case 2: {
+      bool ok;
+      TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);
+      if (!ok) return hipErrorInvalidValue;
+      const int G = persistent_grid(2, device_cu_count(), map.total);
+      if (m % IBM == 0)
+        hipLaunchKernelGGL((oz_igemm_kernel<false>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
+                           map);
+      else
+        hipLaunchKernelGGL((oz_igemm_kernel<true>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
+                           map);
+      return hipGetLastError();
+    }
+    case 3: {
+      switch (npieces_for(nmod)) {
+        case 1:
+        case 2: return launch_recon<2>(tab, count, m, nmod, kc, s);
+        case 3: return launch_recon<3>(tab, count, m, nmod, kc, s);
+        case 4: return launch_recon<4>(tab, count, m, nmod, kc, s);
+        default: return launch_recon<5>(tab, count, m, nmod, kc, s);
+      }
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace caps

@@ -1,0 +1,17 @@
+// Plan choice when the caller passes no pattern (or mode CAPS_MODE_AUTO).
+#pragma once
+#include <string>
+
+#include "../../include/caps_amd.h"
+
+namespace caps {
+
+// One char per level: 'b' (batched) at every level whose workspace still fits
+// the budget, 'd' below.  On one device a 'b' level only costs memory
+// ((7/4)^levels growth), never communication, so batching is preferred.
+std::string choose_pattern(const caps_desc &d, int mode, int moduli, long long budget_doubles);
+
+// Leaf arithmetic when the caller asks for CAPS_MODE_AUTO.
+int choose_mode(const caps_desc &d);
+
+}  // namespace caps

@@ -1,0 +1,164 @@
+/*
+ * caps_amd.h - C ABI of the MI355X (gfx950) CAPS Strassen-Winograd multiply.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point returns
+ * a caps_status (0 = success); caps_last_error() gives a message for the
+ * calling thread's last failure.  Nothing here aborts the process.
+ *
+ * Reference interfaces replaced (paths relative to the CAPS source tree):
+ *   caps_multiply / caps_multiply_device
+ *       -> void multiply(double*, double*, double*, MatDescriptor, char*)
+ *          (multiply.h:13, multiply.cpp:9-25); the C++ overload with that
+ *          exact signature is in include/caps/multiply.h.
+ *   caps_desc        -> struct MatDescriptor (matrix.h:8-16), same 7 ints in
+ *                       the same order (DESC_SIZE = 7, matrix.h:18).
+ *   caps_verify_desc -> verifyDescriptor (matrix.cpp:3-7) for one device.
+ *   caps_num_entries -> numEntriesPerProc (matrix.cpp:9-12).
+ *   caps_to_layout / caps_from_layout (+ _device)
+ *       -> distributeFrom1Proc / collectTo1Proc with one process
+ *          (communication.cpp:612-708, communication.h).
+ *   caps_get_counters / caps_reset_counters
+ *       -> the counters printed by printCounters (counters.cpp:33-74).
+ *   caps_set_available_memory -> setAvailableMemory (memory.cpp:31-33), the
+ *       budget that drives the automatic BFS/DFS choice (memory.cpp:22-29).
+ */
+#ifndef CAPS_AMD_H
+#define CAPS_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same fields, order and size as MatDescriptor (matrix.h:8-16).  On one
+ * device nproc = nprocr = nprocc = nproc_summa = 1. */
+typedef struct caps_desc {
+  int lda;
+  int nrec;
+  int nproc;
+  int nprocr;
+  int nprocc;
+  int nproc_summa;
+  int bs;
+} caps_desc;
+
+typedef enum caps_status {
+  CAPS_OK = 0,
+  CAPS_EINVAL = 1,      /* bad descriptor, size or argument               */
+  CAPS_EPATTERN = 2,    /* pattern char not in [bBdDhH] or too short      */
+  CAPS_ENOMEM = 3,      /* device or host allocation failed / over budget */
+  CAPS_EHIP = 4,        /* a HIP runtime call failed                      */
+  CAPS_ENODEV = 5,      /* no gfx950 device                               */
+  CAPS_EUNSUPPORTED = 6 /* valid request this build does not implement   */
+} caps_status;
+
+/* Leaf (base-case) product arithmetic. */
+typedef enum caps_mode {
+  CAPS_MODE_FP64 = 0,       /* FP64 MFMA leaf GEMM                           */
+  CAPS_MODE_OZAKI_INT8 = 1, /* FP64 operands as scaled integers, split into
+                               residues modulo pairwise-coprime moduli <= 256
+                               (INT8), exact INT8 MFMA products with INT32
+                               accumulation, CRT reconstruction in FP64      */
+  CAPS_MODE_AUTO = 99       /* the planner (autotuner / learned model) picks */
+} caps_mode;
+
+typedef struct caps_opts {
+  int mode;      /* caps_mode                                              */
+  int moduli;    /* split modes: number of moduli (0 = default 14, 2..20)  */
+  int flags;     /* CAPS_FLAG_* below                                      */
+  int reserved;
+} caps_opts;
+
+/* Re-use a captured hipGraph for repeated calls with the same pointers,
+ * descriptor, pattern and options (default on). */
+#define CAPS_FLAG_NO_GRAPH 1
+/* Force the reference's automatic rule when pattern == NULL (DFS at every
+ * level on one process, multiply.cpp:43-66) instead of the planner's choice. */
+#define CAPS_FLAG_REFERENCE_AUTO 2
+/* Run the steps eagerly with a HIP event pair around each launch and add the
+ * elapsed times to the phase timers of caps_get_counters (the reference's
+ * startTimer / stopTimer around each add, dgemm and reorder,
+ * counters.cpp:19-31).  Synchronises the stream before returning. */
+#define CAPS_FLAG_TIMERS 4
+
+/* Select the device (HIP ordinal) for the calling thread and check that it
+ * is gfx950.  Optional: the first call of any entry point does it for the
+ * current device. */
+int caps_init(int device);
+
+/* Message for the calling thread's last failed call ("" if none). */
+const char *caps_last_error(void);
+
+/* verifyDescriptor (matrix.cpp:3-7) for one device, plus bs | leaf size. */
+int caps_verify_desc(const caps_desc *desc);
+
+/* numEntriesPerProc (matrix.cpp:9-12): lda*lda/nproc/nproc_summa. */
+long long caps_num_entries(const caps_desc *desc);
+
+/* C = A * B for n x n FP64 matrices in the CAPS layout (host pointers).
+ * A and B are only read; C is overwritten.  pattern: one char per recursion
+ * level from [bBdDhH] ('b' = batched level, 'd'/'h' = recursive level), or
+ * NULL for the planner's choice.  opts may be NULL (FP64 mode).
+ * multiply.h:13 / multiply.cpp:9-25. */
+int caps_multiply(const double *A, const double *B, double *C,
+                  const caps_desc *desc, const char *pattern,
+                  const caps_opts *opts);
+
+/* Same on device-resident buffers, enqueued on `stream` (a hipStream_t, NULL
+ * = the default stream).  Returns after enqueueing; the caller synchronises.
+ * Device workspace is owned by the library and cached per plan. */
+int caps_multiply_device(const double *dA, const double *dB, double *dC,
+                         const caps_desc *desc, const char *pattern,
+                         const caps_opts *opts, void *stream);
+
+/* Column-major n x n <-> CAPS layout, one process
+ * (distributeFrom1Proc / collectTo1Proc, communication.cpp:612-708). */
+int caps_to_layout(const double *colmajor, double *caps, const caps_desc *desc);
+int caps_from_layout(const double *caps, double *colmajor, const caps_desc *desc);
+int caps_to_layout_device(const double *dcolmajor, double *dcaps,
+                          const caps_desc *desc, void *stream);
+int caps_from_layout_device(const double *dcaps, double *dcolmajor,
+                            const caps_desc *desc, void *stream);
+
+/* The reference's counters (counters.cpp:6-12, 33-47): additions counts one
+ * per element of every add kernel plus m^3 per leaf; multiplications m^3
+ * per leaf (multiply.cpp:1085-1086). */
+/* Phase timers, in the reference's order (counters.h:13-18) plus the split
+ * mode's own phases. */
+enum {
+  CAPS_TIMER_COMM = 0,      /* always 0 on one device                     */
+  CAPS_TIMER_ADD = 1,       /* Winograd pre-additions and combinations    */
+  CAPS_TIMER_MUL = 2,       /* leaf GEMM launches                         */
+  CAPS_TIMER_REARRANGE = 3, /* bs-block <-> column-major leaf reorders    */
+  CAPS_TIMER_COMM_SUMMA = 4,/* always 0 on one device                     */
+  CAPS_TIMER_SPLIT = 5,     /* split mode: scaling and residue splitting  */
+  CAPS_TIMER_RECON = 6,     /* split mode: CRT reconstruction             */
+  CAPS_NUM_TIMERS = 8
+};
+
+typedef struct caps_counters {
+  long long additions;
+  long long multiplications;
+  long long leaf_products;   /* number of base-case products               */
+  double peak_workspace_mb;  /* device workspace of the last plan, MiB     */
+  char pattern[256];         /* "BFS,DFS,..." like setExecutionType        */
+  double timers_ms[CAPS_NUM_TIMERS]; /* accumulated under CAPS_FLAG_TIMERS */
+  long long timed_launches[CAPS_NUM_TIMERS];
+} caps_counters;
+
+int caps_get_counters(caps_counters *out);
+int caps_reset_counters(void);
+
+/* Memory budget in doubles for the automatic BFS/DFS choice
+ * (setAvailableMemory, memory.cpp:31-33; default 2^28 doubles). */
+int caps_set_available_memory(long long doubles);
+
+/* Release cached plans, graphs and workspace. */
+int caps_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CAPS_AMD_H */
