@@ -5,9 +5,12 @@ FP64, with the accuracy gates of the metric.
 Workload (one "step" = one multiply of device-resident CAPS-layout inputs):
   n = 8192, nrec = 2, pattern "bb" (two batched levels: 49 leaf products of
   2048^2 in one launch), bs = 1; A, B entries u * 2^e, u uniform [-1, 1),
-  e uniform integer in [-10, 10] (fixed seeds).  The split run is the
-  reported value; the plain FP64 run of the same multiply is reported beside
-  it with the ratio.  After timing:
+  e uniform integer in [-10, 10] (fixed seeds).  The split run (residue-
+  domain Strassen: INT8 residues of the scaled operands, exact Winograd
+  additions modulo each modulus, INT8 MFMA leaf products, one CRT
+  reconstruction) is the reported value; the plain FP64 run of the same
+  multiply and the leaf-only split (FP64 additions) are reported beside it.
+  After timing:
     * normwise error max|C - C*| / max|C*| on sampled rows against the
       compensated (Dot2) classical product from the CPU oracle,
     * an integer pair in [-64, 64] through the split path checked exactly
@@ -150,24 +153,32 @@ def main():
     dB = make_inputs(torch, n, 5678)
     dC_split = torch.empty_like(dA)
     dC_plain = torch.empty_like(dA)
+    dC_leaf = torch.empty_like(dA)
+    SPLIT = caps_amd.MODE_CRT_INT8
 
     def run_split():
-        caps_amd.multiply_device(dA, dB, dC_split, desc, pat, mode=caps_amd.MODE_OZAKI_INT8,
-                                 moduli=K)
+        caps_amd.multiply_device(dA, dB, dC_split, desc, pat, mode=SPLIT, moduli=K)
 
     def run_plain():
         caps_amd.multiply_device(dA, dB, dC_plain, desc, pat, mode=caps_amd.MODE_FP64)
 
-    log("[bench] split precision (INT8 residues, CRT)")
+    def run_leaf():
+        caps_amd.multiply_device(dA, dB, dC_leaf, desc, pat, mode=caps_amd.MODE_OZAKI_INT8,
+                                 moduli=K)
+
+    log("[bench] split precision (residue-domain Strassen, INT8 MFMA, CRT)")
     t_split = time_steps(torch, run_split, args.steps, args.warmup)
     log(f"[bench] split {t_split * 1e3:.3f} ms/step")
     log("[bench] plain FP64")
     t_plain = time_steps(torch, run_plain, args.steps, args.warmup)
     log(f"[bench] plain {t_plain * 1e3:.3f} ms/step")
+    log("[bench] leaf-only split (FP64 Winograd additions)")
+    t_leaf = time_steps(torch, run_leaf, max(2, args.steps // 2), 1)
+    log(f"[bench] leaf split {t_leaf * 1e3:.3f} ms/step")
 
     # roofline of the dominant kernel: the leaf GEMM launch (all 49 x K INT8 GEMMs)
     per_launch, per_call = leaf_kernel_ms(caps_amd, torch, dA, dB, dC_split, desc, pat,
-                                          caps_amd.MODE_OZAKI_INT8, K)
+                                          SPLIT, K)
     m = n >> r
     leaves = 7 ** r
     int_ops = K * 2.0 * leaves * m ** 3
@@ -195,7 +206,9 @@ def main():
         "config": {"workload": f"CAPS multiply n={n} nrec={r} pattern '{pat}' bs=1, "
                                f"{leaves} leaf products of {m}^2 per launch",
                    "n": n, "nrec": r, "pattern": pat, "moduli": K,
+                   "split_mode": "residue-domain Strassen (CAPS_MODE_CRT_INT8)",
                    "plain_fp64_gflops": round(eff / t_plain / 1e9, 2),
+                   "leaf_split_gflops": round(eff / t_leaf / 1e9, 2),
                    "plain_fp64_ms_per_step": round(t_plain * 1e3, 4),
                    "split_over_plain": round(t_plain / t_split, 3),
                    "phases_ms_split": {k: round(v, 4) for k, v in per_call.items() if v},
@@ -221,13 +234,15 @@ def main():
                                   oracle.from_caps(dC_split.cpu().numpy(), n, r, 1), rows)
         err_plain = sampled_error(oracle, A_cm, B_cm,
                                   oracle.from_caps(dC_plain.cpu().numpy(), n, r, 1), rows)
-        log(f"[bench] error split {err_split:.3e} plain {err_plain:.3e}")
+        err_leaf = sampled_error(oracle, A_cm, B_cm,
+                                 oracle.from_caps(dC_leaf.cpu().numpy(), n, r, 1), rows)
+        log(f"[bench] error split {err_split:.3e} plain {err_plain:.3e} leaf {err_leaf:.3e}")
         del A_cm, B_cm
         log("[bench] integer pair through the split path")
         iA = make_inputs(torch, n, 4321, integer=True)
         iB = make_inputs(torch, n, 8765, integer=True)
         iC = torch.empty_like(iA)
-        caps_amd.multiply_device(iA, iB, iC, desc, pat, mode=caps_amd.MODE_OZAKI_INT8, moduli=K)
+        caps_amd.multiply_device(iA, iB, iC, desc, pat, mode=SPLIT, moduli=K)
         torch.cuda.synchronize()
         exact = exact_integer_check(oracle.from_caps(iA.cpu().numpy(), n, r, 1),
                                     oracle.from_caps(iB.cpu().numpy(), n, r, 1),
@@ -235,6 +250,7 @@ def main():
         log(f"[bench] integer pair bit-exact: {exact}")
         result["config"]["error_split"] = err_split
         result["config"]["error_plain"] = err_plain
+        result["config"]["error_leaf_split"] = err_leaf
         result["config"]["error_metric"] = ("max|C-C*|/max|C*| over 16 sampled full rows, "
                                             "C* = Dot2 compensated classical product")
         result["config"]["integer_pair_bit_exact"] = exact

@@ -32,6 +32,7 @@ SEVEN = 7  # library.h:6
 
 MODE_FP64 = 0
 MODE_OZAKI_INT8 = 1
+MODE_CRT_INT8 = 2
 MODE_AUTO = 99
 FLAG_NO_GRAPH = 1
 FLAG_REFERENCE_AUTO = 2

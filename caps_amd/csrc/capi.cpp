@@ -148,10 +148,12 @@ int resolve_mode(const caps_desc &d, const caps_opts *o, int &mode, int &moduli)
   mode = o ? o->mode : CAPS_MODE_FP64;
   moduli = (o && o->moduli) ? o->moduli : OZ_DEFAULT_MODULI;
   if (mode == CAPS_MODE_AUTO) mode = choose_mode(d);
-  if (mode != CAPS_MODE_FP64 && mode != CAPS_MODE_OZAKI_INT8)
+  if (mode != CAPS_MODE_FP64 && mode != CAPS_MODE_OZAKI_INT8 && mode != CAPS_MODE_CRT_INT8)
     return fail(CAPS_EINVAL, "unknown mode %d", mode);
-  if (mode == CAPS_MODE_OZAKI_INT8 && (moduli < 2 || moduli > OZ_MAX_MODULI))
+  if (mode != CAPS_MODE_FP64 && (moduli < 2 || moduli > OZ_MAX_MODULI))
     return fail(CAPS_EINVAL, "moduli=%d outside 2..%d", moduli, OZ_MAX_MODULI);
+  if (mode == CAPS_MODE_CRT_INT8 && d.nrec > 7)
+    return fail(CAPS_EUNSUPPORTED, "residue-domain mode supports nrec <= 7 (got %d)", d.nrec);
   if (mode == CAPS_MODE_FP64) moduli = 0;
   return CAPS_OK;
 }

@@ -1,0 +1,389 @@
+// Kernels of the residue-domain Strassen (crt.h): scale, split, residue
+// Winograd steps and the final CRT reconstruction.  All HBM-bound.
+#include "crt.h"
+#include "ew_ops.h"
+#include "kernels.h"
+#include "ozaki_dev.h"
+
+namespace caps {
+
+namespace {
+
+constexpr int T = 256;
+
+// ------------------------------------------------------------------ scale
+// partial (e, s) of row i of A over columns [lj0, lj1) of leaf column block bj
+__global__ __launch_bounds__(T) void crt_rowpart_kernel(const CrtMats *__restrict__ M) {
+  const int n = M->n, m = M->m, nrec = M->nrec, bs = M->bs, ch = M->chunks;
+  const int i = blockIdx.x * T + threadIdx.x;
+  if (i >= n) return;
+  const int c = blockIdx.y;  // bj * chunks + k
+  const int bj = c / ch, k = c - bj * ch;
+  const int per = (m + ch - 1) / ch;
+  const int lj0 = k * per, lj1 = min(m, lj0 + per);
+  int e = NOSCALE;
+  double s = 0.0;
+  for (int lj = lj0; lj < lj1; lj++)
+    sq_acc(M->A[caps_elem_offset(n, nrec, bs, i, bj * m + lj)], e, s);
+  reinterpret_cast<double2 *>(M->pa)[(long long)c * n + i] = make_double2(s, (double)e);
+}
+
+// partial of column j of B over leaf row block bi: one wave, fixed butterfly
+__global__ __launch_bounds__(T) void crt_colpart_kernel(const CrtMats *__restrict__ M) {
+  const int n = M->n, m = M->m, nrec = M->nrec, bs = M->bs;
+  const long long w = (long long)blockIdx.x * (T / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int nblk = 1 << nrec;
+  if (w >= (long long)n * nblk) return;
+  const int j = (int)(w % n), bi = (int)(w / n);
+  int e = NOSCALE;
+  double s = 0.0;
+  for (int li = lane; li < m; li += 64)
+    sq_acc(M->B[caps_elem_offset(n, nrec, bs, bi * m + li, j)], e, s);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    int e2 = __shfl_down(e, off, 64);
+    double s2 = __shfl_down(s, off, 64);
+    if (lane < off) sq_merge(e, s, e2, s2);
+  }
+  if (lane == 0) reinterpret_cast<double2 *>(M->pb)[(long long)bi * n + j] = make_double2(s, (double)e);
+}
+
+__global__ __launch_bounds__(T) void crt_finalize_kernel(const CrtMats *__restrict__ M,
+                                                         const OzakiConsts *__restrict__ kc) {
+  const int n = M->n;
+  const int i = blockIdx.x * T + threadIdx.x;
+  if (i >= n) return;
+  const int nblk = 1 << M->nrec;
+  const double2 *pa = reinterpret_cast<const double2 *>(M->pa);
+  const double2 *pb = reinterpret_cast<const double2 *>(M->pb);
+  int e = NOSCALE;
+  double s = 0.0;
+  for (int c = 0; c < nblk * M->chunks; c++) {
+    double2 v = pa[(long long)c * n + i];
+    sq_merge(e, s, (int)v.y, v.x);
+  }
+  M->sa[i] = scale_exponent(e, s, kc->L);
+  e = NOSCALE;
+  s = 0.0;
+  for (int b = 0; b < nblk; b++) {
+    double2 v = pb[(long long)b * n + i];
+    sq_merge(e, s, (int)v.y, v.x);
+  }
+  M->sb[i] = scale_exponent(e, s, kc->L);
+}
+
+// ------------------------------------------------------------------ split
+// A leaf L -> row-major residue leaves; 64 x 64 tiles transposed through LDS.
+// FAST: m % 64 == 0 and the leaf is column-major (bs == 1 or bs == m).
+template <bool FAST>
+__global__ __launch_bounds__(T) void crt_split_a_kernel(const CrtMats *__restrict__ M,
+                                                        const OzakiConsts *__restrict__ kc) {
+  __shared__ double Ts[64][65];  // [col][row]
+  const int m = M->m, nmod = M->nmod, bs = M->bs;
+  const int L = blockIdx.y;
+  int bi, bj;
+  leaf_coords(L, M->nrec, bi, bj);
+  const long long lbase = (long long)L * m * m;
+  const int tiles = (m + 63) / 64;
+  const int r0 = (blockIdx.x % tiles) * 64, k0 = (blockIdx.x / tiles) * 64;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += T) {
+    const int r = idx & 63, c = idx >> 6;
+    const int li = r0 + r, lj = k0 + c;
+    double v = 0.0;
+    if (FAST)
+      v = M->A[lbase + li + (long long)lj * m];
+    else if (li < m && lj < m)
+      v = M->A[lbase + leaf_elem_offset(m, bs, li, lj)];
+    Ts[c][r] = v;
+  }
+  __syncthreads();
+  const int r = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
+  const int li = r0 + r;
+  if (!FAST && li >= m) return;
+  const int sa = M->sa[bi * m + li];
+  double x[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) x[c] = trunc(ldexp(Ts[kq + c][r], sa));
+  const long long plane = (long long)M->n * M->n;
+  int8_t *dst = M->ra + lbase + (long long)li * m + k0 + kq;
+  for (int l = 0; l < nmod; l++) {
+    const int mi = kc->m[l];
+    const double inv = kc->inv_m[l];
+    if (FAST) {
+      unsigned w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        unsigned v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+          v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv) & 0xff) << (8 * b);
+        w[q] = v;
+      }
+      *reinterpret_cast<uint4 *>(dst + l * plane) = make_uint4(w[0], w[1], w[2], w[3]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+        if (k0 + kq + c < m) dst[l * plane + c] = (int8_t)sym_residue(x[c], mi, inv);
+    }
+  }
+}
+
+// B leaf L -> column-major residue leaves; thread per 16 consecutive rows.
+// FAST: m % 16 == 0 and the leaf is column-major.
+template <bool FAST>
+__global__ __launch_bounds__(T) void crt_split_b_kernel(const CrtMats *__restrict__ M,
+                                                        const OzakiConsts *__restrict__ kc) {
+  const int m = M->m, nmod = M->nmod, bs = M->bs;
+  const int L = blockIdx.y;
+  int bi, bj;
+  leaf_coords(L, M->nrec, bi, bj);
+  const long long lbase = (long long)L * m * m;
+  const long long mm = (long long)m * m;
+  const long long plane = (long long)M->n * M->n;
+  const long long units = FAST ? mm / 16 : mm;
+  for (long long u = (long long)blockIdx.x * T + threadIdx.x; u < units;
+       u += (long long)gridDim.x * T) {
+    if (FAST) {
+      const long long e0 = u * 16;
+      const int lj = (int)(e0 / m);
+      const int s = M->sb[bj * m + lj];
+      double x[16];
+#pragma unroll
+      for (int c = 0; c < 16; c += 2) {
+        double2 v = *reinterpret_cast<const double2 *>(M->B + lbase + e0 + c);
+        x[c] = trunc(ldexp(v.x, s));
+        x[c + 1] = trunc(ldexp(v.y, s));
+      }
+      for (int l = 0; l < nmod; l++) {
+        const int mi = kc->m[l];
+        const double inv = kc->inv_m[l];
+        unsigned w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          unsigned v = 0;
+#pragma unroll
+          for (int b = 0; b < 4; b++)
+            v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv) & 0xff) << (8 * b);
+          w[q] = v;
+        }
+        *reinterpret_cast<uint4 *>(M->rb + l * plane + lbase + e0) =
+            make_uint4(w[0], w[1], w[2], w[3]);
+      }
+    } else {
+      const int li = (int)(u % m), lj = (int)(u / m);
+      const double x = trunc(ldexp(M->B[lbase + leaf_elem_offset(m, bs, li, lj)],
+                                   M->sb[bj * m + lj]));
+      for (int l = 0; l < nmod; l++)
+        M->rb[l * plane + lbase + u] = (int8_t)sym_residue(x, kc->m[l], kc->inv_m[l]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ reconstruct
+// C' residues (column-major leaves) -> C (CAPS layout, bs blocks)
+template <int NP, bool FAST>
+__global__ __launch_bounds__(T) void crt_recon_kernel(const CrtMats *__restrict__ M,
+                                                      const OzakiConsts *__restrict__ kc) {
+  const int m = M->m, nmod = M->nmod, bs = M->bs;
+  const int L = blockIdx.y;
+  int bi, bj;
+  leaf_coords(L, M->nrec, bi, bj);
+  const long long lbase = (long long)L * m * m;
+  const long long mm = (long long)m * m;
+  const long long plane = (long long)M->n * M->n;
+  const int per = FAST ? 4 : 1;
+  const long long units = mm / per;
+  for (long long u = (long long)blockIdx.x * T + threadIdx.x; u < units;
+       u += (long long)gridDim.x * T) {
+    const long long e0 = u * per;
+    const int li = (int)(e0 % m), lj = (int)(e0 / m);
+    const int sb = M->sb[bj * m + lj];
+    if (FAST) {
+      double S[4][NP];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int p = 0; p < NP; p++) S[q][p] = 0.0;
+      for (int l = 0; l < nmod; l++) {
+        const unsigned w = *reinterpret_cast<const unsigned *>(M->rc + l * plane + lbase + e0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) crt_acc<NP>(S[q], (int)(int8_t)((w >> (8 * q)) & 0xff), l, kc);
+      }
+      double out[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        out[q] = ldexp(crt_finish<NP>(S[q], kc), -(M->sa[bi * m + li + q] + sb));
+      double2 *dst = reinterpret_cast<double2 *>(M->C + lbase + e0);
+      dst[0] = make_double2(out[0], out[1]);
+      dst[1] = make_double2(out[2], out[3]);
+    } else {
+      double S[NP];
+#pragma unroll
+      for (int p = 0; p < NP; p++) S[p] = 0.0;
+      for (int l = 0; l < nmod; l++) crt_acc<NP>(S, (int)M->rc[l * plane + lbase + e0], l, kc);
+      M->C[lbase + leaf_elem_offset(m, bs, li, lj)] =
+          ldexp(crt_finish<NP>(S, kc), -(M->sa[bi * m + li] + sb));
+    }
+  }
+}
+
+// ------------------------------------------------------------------ residue Winograd steps
+template <int OP, bool VEC4>
+__global__ __launch_bounds__(T) void rew_kernel(const RPtrs *__restrict__ tab, long long n,
+                                                const OzakiConsts *__restrict__ kc) {
+  using Op = Ew<OP>;
+  const RPtrs &P = tab[blockIdx.y];
+  const int l = blockIdx.z;
+  const int mi = kc->m[l];
+  const float inv = 1.0f / (float)mi;
+  const int8_t *in[Op::NIN];
+  int8_t *out[Op::NOUT];
+#pragma unroll
+  for (int k = 0; k < Op::NIN; k++) in[k] = P.in[k] + l * P.sin[k];
+#pragma unroll
+  for (int k = 0; k < Op::NOUT; k++) out[k] = P.out[k] + l * P.sout[k];
+  const long long stride = (long long)gridDim.x * T;
+  if (VEC4) {
+    const long long n4 = n >> 2;
+    for (long long i = (long long)blockIdx.x * T + threadIdx.x; i < n4; i += stride) {
+      unsigned w[Op::NIN];
+#pragma unroll
+      for (int k = 0; k < Op::NIN; k++) w[k] = reinterpret_cast<const unsigned *>(in[k])[i];
+      unsigned o[Op::NOUT];
+#pragma unroll
+      for (int k = 0; k < Op::NOUT; k++) o[k] = 0;
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        int x[Op::NIN], y[Op::NOUT];
+#pragma unroll
+        for (int k = 0; k < Op::NIN; k++) x[k] = (int)(int8_t)((w[k] >> (8 * b)) & 0xff);
+        Op::apply(x, y);
+#pragma unroll
+        for (int k = 0; k < Op::NOUT; k++)
+          o[k] |= (unsigned)(mod_i32(y[k], mi, inv) & 0xff) << (8 * b);
+      }
+#pragma unroll
+      for (int k = 0; k < Op::NOUT; k++) reinterpret_cast<unsigned *>(out[k])[i] = o[k];
+    }
+  } else {
+    for (long long i = (long long)blockIdx.x * T + threadIdx.x; i < n; i += stride) {
+      int x[Op::NIN], y[Op::NOUT];
+#pragma unroll
+      for (int k = 0; k < Op::NIN; k++) x[k] = (int)in[k][i];
+      Op::apply(x, y);
+#pragma unroll
+      for (int k = 0; k < Op::NOUT; k++) out[k][i] = (int8_t)mod_i32(y[k], mi, inv);
+    }
+  }
+}
+
+template <int OP>
+hipError_t launch_rew_op(const RPtrs *tab, int count, long long n, int nmod, bool vec4,
+                         const OzakiConsts *kc, hipStream_t s) {
+  const long long units = vec4 ? (n >> 2) : n;
+  long long blocks = (units + T - 1) / T;
+  long long cap = (long long)device_cu_count() * 8 / ((long long)count * nmod);
+  if (cap < 1) cap = 1;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  dim3 grid((unsigned)blocks, (unsigned)count, (unsigned)nmod);
+  if (vec4)
+    hipLaunchKernelGGL((rew_kernel<OP, true>), grid, dim3(T), 0, s, tab, n, kc);
+  else
+    hipLaunchKernelGGL((rew_kernel<OP, false>), grid, dim3(T), 0, s, tab, n, kc);
+  return hipGetLastError();
+}
+
+unsigned capped_blocks(long long units, long long per_grid) {
+  long long b = (units + T - 1) / T;
+  long long cap = (long long)device_cu_count() * 8 / (per_grid > 0 ? per_grid : 1);
+  if (cap < 1) cap = 1;
+  if (b > cap) b = cap;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+template <int NP>
+hipError_t launch_recon(const CrtMats *M, const CrtMats &h, const OzakiConsts *kc, hipStream_t s) {
+  const bool fast = (h.m % 4 == 0) && (h.bs == 1 || h.bs == h.m);
+  const long long mm = (long long)h.m * h.m;
+  const int leaves = 1 << (2 * h.nrec);
+  dim3 grid(capped_blocks(fast ? mm / 4 : mm, leaves), leaves);
+  if (fast)
+    hipLaunchKernelGGL((crt_recon_kernel<NP, true>), grid, dim3(T), 0, s, M, kc);
+  else
+    hipLaunchKernelGGL((crt_recon_kernel<NP, false>), grid, dim3(T), 0, s, M, kc);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_crt_step(int step, const CrtMats *M, const CrtMats &h, const OzakiConsts *kc,
+                           hipStream_t s) {
+  const int leaves = 1 << (2 * h.nrec);
+  const int nblk = 1 << h.nrec;
+  if (leaves > 65535) return hipErrorInvalidValue;
+  switch (step) {
+    case 0: {
+      dim3 g1((h.n + T - 1) / T, nblk * h.chunks);
+      hipLaunchKernelGGL(crt_rowpart_kernel, g1, dim3(T), 0, s, M);
+      const long long waves = (long long)h.n * nblk;
+      dim3 g2((unsigned)((waves + 3) / 4));
+      hipLaunchKernelGGL(crt_colpart_kernel, g2, dim3(T), 0, s, M);
+      dim3 g3((h.n + T - 1) / T);
+      hipLaunchKernelGGL(crt_finalize_kernel, g3, dim3(T), 0, s, M, kc);
+      return hipGetLastError();
+    }
+    case 1: {
+      const bool col = (h.bs == 1 || h.bs == h.m);
+      const int tiles = (h.m + 63) / 64;
+      dim3 ga((unsigned)(tiles * tiles), leaves);
+      if (col && h.m % 64 == 0)
+        hipLaunchKernelGGL((crt_split_a_kernel<true>), ga, dim3(T), 0, s, M, kc);
+      else
+        hipLaunchKernelGGL((crt_split_a_kernel<false>), ga, dim3(T), 0, s, M, kc);
+      const long long mm = (long long)h.m * h.m;
+      const bool fb = col && h.m % 16 == 0;
+      dim3 gb(capped_blocks(fb ? mm / 16 : mm, leaves), leaves);
+      if (fb)
+        hipLaunchKernelGGL((crt_split_b_kernel<true>), gb, dim3(T), 0, s, M, kc);
+      else
+        hipLaunchKernelGGL((crt_split_b_kernel<false>), gb, dim3(T), 0, s, M, kc);
+      return hipGetLastError();
+    }
+    case 2: {
+      switch (ozaki_npieces(h.nmod)) {
+        case 1:
+        case 2: return launch_recon<2>(M, h, kc, s);
+        case 3: return launch_recon<3>(M, h, kc, s);
+        default: return launch_recon<4>(M, h, kc, s);
+      }
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_residue_ew(int op, const RPtrs *tab, int count, long long n, int nmod,
+                             bool vec4, const OzakiConsts *kc, hipStream_t s) {
+  if (count <= 0 || n <= 0) return hipSuccess;
+  if (nmod < 2 || nmod > OZ_MAX_MODULI) return hipErrorInvalidValue;
+  for (int c0 = 0; c0 < count; c0 += 65535) {
+    const RPtrs *t = tab + c0;
+    const int c = count - c0 < 65535 ? count - c0 : 65535;
+    hipError_t e = hipErrorInvalidValue;
+    switch (op) {
+      case EW_BFS_PRE: e = launch_rew_op<EW_BFS_PRE>(t, c, n, nmod, vec4, kc, s); break;
+      case EW_BFS_POST: e = launch_rew_op<EW_BFS_POST>(t, c, n, nmod, vec4, kc, s); break;
+      case EW_DFS_PRE1: e = launch_rew_op<EW_DFS_PRE1>(t, c, n, nmod, vec4, kc, s); break;
+      case EW_DFS_PRE2: e = launch_rew_op<EW_DFS_PRE2>(t, c, n, nmod, vec4, kc, s); break;
+      case EW_DFS_PRE3: e = launch_rew_op<EW_DFS_PRE3>(t, c, n, nmod, vec4, kc, s); break;
+      case EW_DFS_POST1: e = launch_rew_op<EW_DFS_POST1>(t, c, n, nmod, vec4, kc, s); break;
+      case EW_DFS_POST2: e = launch_rew_op<EW_DFS_POST2>(t, c, n, nmod, vec4, kc, s); break;
+      case EW_DFS_POST3: e = launch_rew_op<EW_DFS_POST3>(t, c, n, nmod, vec4, kc, s); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace caps

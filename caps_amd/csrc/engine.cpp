@@ -15,6 +15,9 @@
 //   leaf (block_multiply, multiply.cpp:1082-1141): one batched GEMM launch
 //        over every sub-problem of the batch; for 1 < bs < m the leaves are
 //        reordered to column-major around it (1093-1139).
+// The same recursion runs on FP64 matrices (CAPS_MODE_FP64, and
+// CAPS_MODE_OZAKI_INT8 whose leaves are split products) or on INT8 residue
+// planes (CAPS_MODE_CRT_INT8, crt.h), where a matrix is nmod planes.
 // Workspace is a stack: each level allocates on entry and releases on exit,
 // which is safe because every launch is ordered on one stream.
 #include "engine.h"
@@ -22,6 +25,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "crt.h"
 #include "ozaki.h"
 
 namespace caps {
@@ -39,11 +43,13 @@ struct Builder {
   Plan &plan;
   char *base;  // workspace base (nullptr while sizing)
   size_t top = 0, peak = 0;
-  int bs;
-  int mode;
-  int moduli;
+  int bs = 1;
+  int mode = CAPS_MODE_FP64;
+  int moduli = 0;
   std::string pattern;
-  int top_nrec;
+  bool residue = false;  // matrices are nmod INT8 planes
+  int es = 8;            // bytes per element: 8 (FP64) or 1 (residue)
+  int planes = 1;        // 1 or nmod
 
   size_t alloc(size_t bytes) {
     size_t off = top;
@@ -51,7 +57,15 @@ struct Builder {
     peak = std::max(peak, top);
     return off;
   }
-  double *dptr(size_t off) { return reinterpret_cast<double *>(base + off); }
+
+  // `count` matrices of `elems` elements each, one block per plane:
+  // matrix p at base + p * elems * es, plane stride count * elems * es
+  Mat alloc_mats(size_t count, long long elems) {
+    const size_t per_plane = count * (size_t)elems * es;
+    size_t off = alloc(per_plane * planes);
+    return {base + off, (long long)per_plane};
+  }
+  Mat at(Mat M, long long elems) const { return {M.p + elems * es, M.ps}; }
 
   size_t add_table(const void *data, size_t bytes) {
     auto &t = plan.host_tables;
@@ -61,21 +75,69 @@ struct Builder {
     return off;
   }
 
-  void ew(int op, const std::vector<EwPtrs> &tab, long long q) {
+  void ensure_consts() {
+    if (plan.oz_moduli == 0) {
+      OzakiConsts k;
+      ozaki_make_consts(moduli, k);
+      plan.oz_const = add_table(&k, sizeof(k));
+      plan.oz_moduli = moduli;
+    }
+  }
+
+  void push(int kind, int op, int count, long long n, int bs_, size_t table, int vec = 0) {
     Step st{};
-    st.kind = STEP_EW;
+    st.kind = kind;
     st.op = op;
-    st.count = (int)tab.size();
-    st.n = q;
-    // 16-byte accesses need every quadrant offset 16-byte aligned
-    st.vec2 = (q % 2 == 0);
-    for (auto &e : tab)
-      for (int k = 0; k < 8; k++)
-        if ((reinterpret_cast<uintptr_t>(e.in[k]) | reinterpret_cast<uintptr_t>(e.out[k])) & 15)
-          st.vec2 = 0;
-    st.table = add_table(tab.data(), tab.size() * sizeof(EwPtrs));
+    st.count = count;
+    st.n = n;
+    st.bs = bs_;
+    st.vec2 = vec;
+    st.table = table;
     plan.steps.push_back(st);
-    plan.additions += (long long)ew_additions(op) * q * (long long)tab.size();
+  }
+
+  // one Winograd add step over the batch: ins[p][k], outs[p][k]
+  void ew(int op, const std::vector<std::vector<Mat>> &ins,
+          const std::vector<std::vector<Mat>> &outs, long long q) {
+    const int count = (int)ins.size();
+    if (!residue) {
+      std::vector<EwPtrs> t(ins.size());
+      bool aligned = (q % 2 == 0);  // 16-byte accesses
+      for (size_t p = 0; p < ins.size(); p++) {
+        EwPtrs e{};
+        for (size_t k = 0; k < ins[p].size(); k++)
+          e.in[k] = reinterpret_cast<const double *>(ins[p][k].p);
+        for (size_t k = 0; k < outs[p].size(); k++)
+          e.out[k] = reinterpret_cast<double *>(outs[p][k].p);
+        for (int k = 0; k < 8; k++)
+          if ((reinterpret_cast<uintptr_t>(e.in[k]) | reinterpret_cast<uintptr_t>(e.out[k])) & 15)
+            aligned = false;
+        t[p] = e;
+      }
+      push(STEP_EW, op, count, q, 0, add_table(t.data(), t.size() * sizeof(EwPtrs)), aligned);
+    } else {
+      ensure_consts();
+      std::vector<RPtrs> t(ins.size());
+      bool aligned = (q % 4 == 0);  // 4-byte accesses
+      for (size_t p = 0; p < ins.size(); p++) {
+        RPtrs e{};
+        for (size_t k = 0; k < ins[p].size(); k++) {
+          e.in[k] = reinterpret_cast<const int8_t *>(ins[p][k].p);
+          e.sin[k] = ins[p][k].ps;
+        }
+        for (size_t k = 0; k < outs[p].size(); k++) {
+          e.out[k] = reinterpret_cast<int8_t *>(outs[p][k].p);
+          e.sout[k] = outs[p][k].ps;
+        }
+        for (int k = 0; k < 8; k++)
+          if (((reinterpret_cast<uintptr_t>(e.in[k]) | reinterpret_cast<uintptr_t>(e.out[k])) & 3) ||
+              ((e.sin[k] | e.sout[k]) & 3))
+            aligned = false;
+        t[p] = e;
+      }
+      push(STEP_REW, op, count, q, 0, add_table(t.data(), t.size() * sizeof(RPtrs)), aligned);
+    }
+    plan.additions += (long long)ew_additions(op) * q * (long long)ins.size();
   }
 
   void level_tag(int depth, char c) {
@@ -92,160 +154,181 @@ struct Builder {
     plan.additions += m3 * (long long)batch.size();
     plan.multiplications += m3 * (long long)batch.size();
     plan.leaves += (long long)batch.size();
+    if (residue) {
+      // residue leaves are already row-major (A) / column-major (B, C)
+      ensure_consts();
+      std::vector<RGemm> g(batch.size());
+      for (size_t p = 0; p < batch.size(); p++)
+        g[p] = {reinterpret_cast<const int8_t *>(batch[p].A.p),
+                reinterpret_cast<const int8_t *>(batch[p].B.p),
+                reinterpret_cast<int8_t *>(batch[p].C.p), batch[p].A.ps, batch[p].B.ps,
+                batch[p].C.ps};
+      push(STEP_OZ_GEMM, moduli, (int)g.size(), m, 0,
+           add_table(g.data(), g.size() * sizeof(RGemm)));
+      return;
+    }
     const size_t mark = top;
     std::vector<Prob> work = batch;
     const bool reorder = (bs != 1 && bs != m);
-    const size_t msz = (size_t)m * m * sizeof(double);
+    const long long mm = (long long)m * m;
     if (reorder) {
-      size_t AA = alloc(msz * batch.size()), BB = alloc(msz * batch.size()),
-             CC = alloc(msz * batch.size());
-      std::vector<ReorderPtrs> ra, rb;
+      Mat AA = alloc_mats(batch.size(), mm), BB = alloc_mats(batch.size(), mm),
+          CC = alloc_mats(batch.size(), mm);
+      std::vector<ReorderPtrs> rr;
       for (size_t p = 0; p < batch.size(); p++) {
-        work[p].A = dptr(AA + p * msz);
-        work[p].B = dptr(BB + p * msz);
-        work[p].C = dptr(CC + p * msz);
-        ra.push_back({batch[p].A, const_cast<double *>(work[p].A)});
-        rb.push_back({batch[p].B, const_cast<double *>(work[p].B)});
+        work[p].A = at(AA, (long long)p * mm);
+        work[p].B = at(BB, (long long)p * mm);
+        work[p].C = at(CC, (long long)p * mm);
+        rr.push_back({reinterpret_cast<const double *>(batch[p].A.p),
+                      reinterpret_cast<double *>(work[p].A.p)});
       }
-      ra.insert(ra.end(), rb.begin(), rb.end());
-      Step st{};
-      st.kind = STEP_REORDER;
-      st.op = 1;
-      st.count = (int)ra.size();
-      st.n = m;
-      st.bs = bs;
-      st.table = add_table(ra.data(), ra.size() * sizeof(ReorderPtrs));
-      plan.steps.push_back(st);
+      for (size_t p = 0; p < batch.size(); p++)
+        rr.push_back({reinterpret_cast<const double *>(batch[p].B.p),
+                      reinterpret_cast<double *>(work[p].B.p)});
+      push(STEP_REORDER, 1, (int)rr.size(), m, bs,
+           add_table(rr.data(), rr.size() * sizeof(ReorderPtrs)));
     }
     if (mode == CAPS_MODE_FP64) {
       std::vector<GemmPtrs> g;
-      for (auto &p : work) g.push_back({p.A, p.B, p.C});
-      Step st{};
-      st.kind = STEP_GEMM_F64;
-      st.count = (int)g.size();
-      st.n = m;
-      st.table = add_table(g.data(), g.size() * sizeof(GemmPtrs));
-      plan.steps.push_back(st);
+      for (auto &p : work)
+        g.push_back({reinterpret_cast<const double *>(p.A.p),
+                     reinterpret_cast<const double *>(p.B.p), reinterpret_cast<double *>(p.C.p)});
+      push(STEP_GEMM_F64, 0, (int)g.size(), m, 0, add_table(g.data(), g.size() * sizeof(GemmPtrs)));
     } else {
       ozaki_leaf_steps(work, m);
     }
     if (reorder) {
       std::vector<ReorderPtrs> rc;
-      for (size_t p = 0; p < batch.size(); p++) rc.push_back({work[p].C, batch[p].C});
-      Step st{};
-      st.kind = STEP_REORDER;
-      st.op = 0;
-      st.count = (int)rc.size();
-      st.n = m;
-      st.bs = bs;
-      st.table = add_table(rc.data(), rc.size() * sizeof(ReorderPtrs));
-      plan.steps.push_back(st);
+      for (size_t p = 0; p < batch.size(); p++)
+        rc.push_back({reinterpret_cast<const double *>(work[p].C.p),
+                      reinterpret_cast<double *>(batch[p].C.p)});
+      push(STEP_REORDER, 0, (int)rc.size(), m, bs,
+           add_table(rc.data(), rc.size() * sizeof(ReorderPtrs)));
     }
     top = mark;
   }
 
-  // ---------------------------------------------------------------- split mode
-  // Leaf product of a batch through the Ozaki-II CRT scheme (ozaki.h).
-  void ozaki_leaf_steps(const std::vector<Prob> &work, int m);
+  // Leaf-only split (CAPS_MODE_OZAKI_INT8): FP64 leaf product through the
+  // Ozaki-II CRT scheme (ozaki.h).
+  void ozaki_leaf_steps(const std::vector<Prob> &work, int m) {
+    const int nmod = moduli;
+    const size_t nb = work.size();
+    const size_t plane = (size_t)m * m;  // bytes per INT8 plane
+    ensure_consts();
+    const size_t ex_bytes = (size_t)m * sizeof(int);
+    const size_t pa_bytes = (size_t)OZ_CHUNKS * m * 2 * sizeof(double);
+    const size_t pb_bytes = (size_t)m * 2 * sizeof(double);
+    const size_t mark = top;
+    size_t EA = alloc(ex_bytes * nb), EB = alloc(ex_bytes * nb);
+    size_t PA = alloc(pa_bytes * nb), PB = alloc(pb_bytes * nb);
+    size_t RA = alloc(plane * nmod * nb), RB = alloc(plane * nmod * nb);
+    size_t RC = alloc(plane * nmod * nb);
+    std::vector<OzLeaf> t(nb);
+    std::vector<RGemm> g(nb);
+    for (size_t p = 0; p < nb; p++) {
+      OzLeaf &L = t[p];
+      L.A = reinterpret_cast<const double *>(work[p].A.p);
+      L.B = reinterpret_cast<const double *>(work[p].B.p);
+      L.C = reinterpret_cast<double *>(work[p].C.p);
+      L.ea = reinterpret_cast<int *>(base + EA + ex_bytes * p);
+      L.eb = reinterpret_cast<int *>(base + EB + ex_bytes * p);
+      L.pa = reinterpret_cast<double *>(base + PA + pa_bytes * p);
+      L.pb = reinterpret_cast<double *>(base + PB + pb_bytes * p);
+      L.ra = reinterpret_cast<int8_t *>(base + RA + plane * nmod * p);
+      L.rb = reinterpret_cast<int8_t *>(base + RB + plane * nmod * p);
+      L.rc = reinterpret_cast<int8_t *>(base + RC + plane * nmod * p);
+      g[p] = {L.ra, L.rb, L.rc, (long long)plane, (long long)plane, (long long)plane};
+    }
+    const size_t tab = add_table(t.data(), t.size() * sizeof(OzLeaf));
+    const size_t gtab = add_table(g.data(), g.size() * sizeof(RGemm));
+    push(STEP_OZ_SCALE, nmod, (int)nb, m, 0, tab);
+    push(STEP_OZ_SPLIT, nmod, (int)nb, m, 0, tab);
+    push(STEP_OZ_GEMM, nmod, (int)nb, m, 0, gtab);
+    push(STEP_OZ_RECON, nmod, (int)nb, m, 0, tab);
+    top = mark;
+  }
 
   void bfs(const std::vector<Prob> &batch, int s, int nrec, int depth) {
     const int h = s / 2;
     const long long q = (long long)h * h;
-    const size_t qb = (size_t)q * sizeof(double);
     const size_t mark = top;
     const size_t nb = batch.size();
-    size_t LF = alloc(4 * qb * nb), RF = alloc(4 * qb * nb), P = alloc(7 * qb * nb);
-    std::vector<EwPtrs> pre(nb), post(nb);
+    Mat LF = alloc_mats(4 * nb, q), RF = alloc_mats(4 * nb, q), P = alloc_mats(7 * nb, q);
+    std::vector<std::vector<Mat>> pre_in(nb), pre_out(nb), post_in(nb), post_out(nb);
     std::vector<Prob> next;
     next.reserve(7 * nb);
     for (size_t p = 0; p < nb; p++) {
-      const double *A = batch[p].A, *B = batch[p].B;
-      double *C = batch[p].C;
-      double *lf = dptr(LF + 4 * qb * p), *rf = dptr(RF + 4 * qb * p), *pp = dptr(P + 7 * qb * p);
-      EwPtrs &e = pre[p];
-      for (int k = 0; k < 4; k++) {
-        e.in[k] = A + k * q;          // A11 A21 A12 A22
-        e.in[4 + k] = B + k * q;      // B11 B21 B12 B22
-        e.out[k] = lf + k * q;        // T3 T4 T5 T6
-        e.out[4 + k] = rf + k * q;    // S3 S4 S5 S7
-      }
-      const double *T[7] = {A, A + 2 * q, lf, lf + q, lf + 2 * q, lf + 3 * q, A + 3 * q};
-      const double *S[7] = {B, B + q, rf, rf + q, rf + 2 * q, B + 3 * q, rf + 3 * q};
-      for (int i = 0; i < 7; i++) next.push_back({T[i], S[i], pp + i * q});
-      EwPtrs &o = post[p];
-      for (int i = 0; i < 7; i++) o.in[i] = pp + i * q;
-      o.in[7] = nullptr;
-      for (int k = 0; k < 4; k++) o.out[k] = C + k * q;  // C11 C21 C12 C22
-      for (int k = 4; k < 8; k++) o.out[k] = nullptr;
+      const Mat A = batch[p].A, B = batch[p].B, C = batch[p].C;
+      const Mat lf = at(LF, 4 * q * (long long)p), rf = at(RF, 4 * q * (long long)p),
+                pp = at(P, 7 * q * (long long)p);
+      for (int k = 0; k < 4; k++) pre_in[p].push_back(at(A, k * q));    // A11 A21 A12 A22
+      for (int k = 0; k < 4; k++) pre_in[p].push_back(at(B, k * q));    // B11 B21 B12 B22
+      for (int k = 0; k < 4; k++) pre_out[p].push_back(at(lf, k * q));  // T3 T4 T5 T6
+      for (int k = 0; k < 4; k++) pre_out[p].push_back(at(rf, k * q));  // S3 S4 S5 S7
+      const Mat T[7] = {A, at(A, 2 * q), lf, at(lf, q), at(lf, 2 * q), at(lf, 3 * q), at(A, 3 * q)};
+      const Mat S[7] = {B, at(B, q), rf, at(rf, q), at(rf, 2 * q), at(B, 3 * q), at(rf, 3 * q)};
+      for (int i = 0; i < 7; i++) next.push_back({T[i], S[i], at(pp, i * q)});
+      for (int i = 0; i < 7; i++) post_in[p].push_back(at(pp, i * q));
+      for (int k = 0; k < 4; k++) post_out[p].push_back(at(C, k * q));  // C11 C21 C12 C22
     }
-    ew(EW_BFS_PRE, pre, q);
+    ew(EW_BFS_PRE, pre_in, pre_out, q);
     rec(next, h, nrec - 1, depth + 1);
-    ew(EW_BFS_POST, post, q);
+    ew(EW_BFS_POST, post_in, post_out, q);
     top = mark;
   }
 
   void dfs(const std::vector<Prob> &batch, int s, int nrec, int depth) {
     const int h = s / 2;
     const long long q = (long long)h * h;
-    const size_t qb = (size_t)q * sizeof(double);
     const size_t mark = top;
     const size_t nb = batch.size();
-    size_t R2o = alloc(qb * nb), R5o = alloc(qb * nb);
+    Mat R2o = alloc_mats(nb, q), R5o = alloc_mats(nb, q);
     struct Regs {
-      const double *A11, *A21, *A12, *A22, *B11, *B21, *B12, *B22;
-      double *R1, *R2, *R3, *R4, *R5, *R6;
+      Mat A11, A21, A12, A22, B11, B21, B12, B22;
+      Mat R1, R2, R3, R4, R5, R6;
     };
     std::vector<Regs> r(nb);
     for (size_t p = 0; p < nb; p++) {
-      const double *A = batch[p].A, *B = batch[p].B;
-      double *C = batch[p].C;
-      r[p] = {A, A + q, A + 2 * q, A + 3 * q, B, B + q, B + 2 * q, B + 3 * q,
-              C + q, dptr(R2o + qb * p), C, C + 3 * q, dptr(R5o + qb * p), C + 2 * q};
+      const Mat A = batch[p].A, B = batch[p].B, C = batch[p].C;
+      r[p] = {A,        at(A, q),  at(A, 2 * q), at(A, 3 * q),
+              B,        at(B, q),  at(B, 2 * q), at(B, 3 * q),
+              at(C, q), at(R2o, q * (long long)p), C, at(C, 3 * q),
+              at(R5o, q * (long long)p), at(C, 2 * q)};
     }
-    auto table = [&](auto fill) {
-      std::vector<EwPtrs> t(nb);
+    using V = std::vector<Mat>;
+    auto step = [&](int op, V (*pick_in)(const Regs &), V (*pick_out)(const Regs &)) {
+      std::vector<V> ins(nb), outs(nb);
       for (size_t p = 0; p < nb; p++) {
-        EwPtrs e{};
-        fill(r[p], e);
-        t[p] = e;
+        ins[p] = pick_in(r[p]);
+        outs[p] = pick_out(r[p]);
       }
-      return t;
+      ew(op, ins, outs, q);
     };
-    auto sub = [&](auto pick) {
+    auto sub = [&](Prob (*pick)(const Regs &)) {
       std::vector<Prob> b(nb);
       for (size_t p = 0; p < nb; p++) b[p] = pick(r[p]);
       rec(b, h, nrec - 1, depth + 1);
     };
-    // S5=B22-B12 (R1), S3=B12-B11 (R2), S4=B22-S3 (R3); T3=A21+A22 (R6), T5=A11-A21 (R4)
-    ew(EW_DFS_PRE1, table([](const Regs &x, EwPtrs &e) {
-         e.in[0] = x.A11; e.in[1] = x.A21; e.in[2] = x.A22;
-         e.in[3] = x.B11; e.in[4] = x.B12; e.in[5] = x.B22;
-         e.out[0] = x.R1; e.out[1] = x.R2; e.out[2] = x.R3; e.out[3] = x.R4; e.out[4] = x.R6;
-       }), q);
+    // S5=B22-B12 (R1), S3=B12-B11 (R2), S4=B22-S3 (R3); T5=A11-A21 (R4), T3=A21+A22 (R6)
+    step(EW_DFS_PRE1, [](const Regs &x) { return V{x.A11, x.A21, x.A22, x.B11, x.B12, x.B22}; },
+         [](const Regs &x) { return V{x.R1, x.R2, x.R3, x.R4, x.R6}; });
     sub([](const Regs &x) { return Prob{x.R4, x.R1, x.R5}; });   // Q5 = T5 S5
     sub([](const Regs &x) { return Prob{x.R6, x.R2, x.R4}; });   // Q3 = T3 S3
-    ew(EW_DFS_PRE2, table([](const Regs &x, EwPtrs &e) {       // T4 = T3 - A11
-         e.in[0] = x.R6; e.in[1] = x.A11; e.out[0] = x.R6;
-       }), q);
+    step(EW_DFS_PRE2, [](const Regs &x) { return V{x.R6, x.A11}; },  // T4 = T3 - A11
+         [](const Regs &x) { return V{x.R6}; });
     sub([](const Regs &x) { return Prob{x.R6, x.R3, x.R2}; });   // Q4 = T4 S4
-    ew(EW_DFS_PRE3, table([](const Regs &x, EwPtrs &e) {       // T6 = A12-T4, S7 = S4-B21
-         e.in[0] = x.A12; e.in[1] = x.R6; e.in[2] = x.R3; e.in[3] = x.B21;
-         e.out[0] = x.R6; e.out[1] = x.R3;
-       }), q);
+    step(EW_DFS_PRE3, [](const Regs &x) { return V{x.A12, x.R6, x.R3, x.B21}; },  // T6, S7
+         [](const Regs &x) { return V{x.R6, x.R3}; });
     sub([](const Regs &x) { return Prob{x.A22, x.R3, x.R1}; });  // Q7 = A22 S7
     sub([](const Regs &x) { return Prob{x.A11, x.B11, x.R3}; }); // Q1 = A11 B11
-    ew(EW_DFS_POST1, table([](const Regs &x, EwPtrs &e) {      // U's, C22, C21
-         e.in[0] = x.R3; e.in[1] = x.R2; e.in[2] = x.R5; e.in[3] = x.R4; e.in[4] = x.R1;
-         e.out[0] = x.R2; e.out[1] = x.R4; e.out[2] = x.R1;
-       }), q);
+    step(EW_DFS_POST1, [](const Regs &x) { return V{x.R3, x.R2, x.R5, x.R4, x.R1}; },  // U3 C22 C21
+         [](const Regs &x) { return V{x.R2, x.R4, x.R1}; });
     sub([](const Regs &x) { return Prob{x.A12, x.B21, x.R5}; }); // Q2 = A12 B21
-    ew(EW_DFS_POST2, table([](const Regs &x, EwPtrs &e) {      // C11 = Q1 + Q2
-         e.in[0] = x.R3; e.in[1] = x.R5; e.out[0] = x.R3;
-       }), q);
+    step(EW_DFS_POST2, [](const Regs &x) { return V{x.R3, x.R5}; },  // C11 = Q1 + Q2
+         [](const Regs &x) { return V{x.R3}; });
     sub([](const Regs &x) { return Prob{x.R6, x.B22, x.R5}; });  // Q6 = T6 B22
-    ew(EW_DFS_POST3, table([](const Regs &x, EwPtrs &e) {      // C12 = U3 + Q6
-         e.in[0] = x.R2; e.in[1] = x.R5; e.out[0] = x.R6;
-       }), q);
+    step(EW_DFS_POST3, [](const Regs &x) { return V{x.R2, x.R5}; },  // C12 = U3 + Q6
+         [](const Regs &x) { return V{x.R6}; });
     top = mark;
   }
 
@@ -261,55 +344,98 @@ struct Builder {
     else
       dfs(batch, s, nrec, depth);
   }
+
+  // residue mode: scale and split A, B once, recurse on residues, reconstruct
+  void crt_top(const double *A, const double *B, double *C, const caps_desc &d) {
+    const int n = d.lda, m = n >> d.nrec;
+    const long long n2 = (long long)n * n;
+    ensure_consts();
+    CrtMats M{};
+    M.A = A;
+    M.B = B;
+    M.C = C;
+    M.n = n;
+    M.nrec = d.nrec;
+    M.bs = d.bs;
+    M.m = m;
+    M.nmod = moduli;
+    M.chunks = crt_row_chunks(m);
+    const int nblk = 1 << d.nrec;
+    M.sa = reinterpret_cast<int *>(base + alloc(sizeof(int) * n));
+    M.sb = reinterpret_cast<int *>(base + alloc(sizeof(int) * n));
+    M.pa = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk * M.chunks));
+    M.pb = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk));
+    Mat RA = alloc_mats(1, n2), RB = alloc_mats(1, n2), RC = alloc_mats(1, n2);
+    M.ra = reinterpret_cast<int8_t *>(RA.p);
+    M.rb = reinterpret_cast<int8_t *>(RB.p);
+    M.rc = reinterpret_cast<int8_t *>(RC.p);
+    const size_t mt = add_table(&M, sizeof(M));
+    plan.crt_mats = mt;
+    plan.crt_host.assign(reinterpret_cast<unsigned char *>(&M),
+                         reinterpret_cast<unsigned char *>(&M) + sizeof(M));
+    push(STEP_CRT_SCALE, moduli, 1, n, d.bs, mt);
+    push(STEP_CRT_SPLIT, moduli, 1, n, d.bs, mt);
+    std::vector<Prob> top_batch{{RA, RB, RC}};
+    rec(top_batch, n, d.nrec, 0);
+    push(STEP_CRT_RECON, moduli, 1, n, d.bs, mt);
+  }
 };
 
-}  // namespace
-
-// Split-mode leaf: workspace and tables for the Ozaki-II steps (ozaki.h).
-void Builder::ozaki_leaf_steps(const std::vector<Prob> &work, int m) {
-  const int nmod = moduli;
-  const size_t nb = work.size();
-  const size_t plane = (size_t)m * m;  // bytes per INT8 plane
-  if (plan.oz_moduli == 0) {
-    OzakiConsts k;
-    ozaki_make_consts(nmod, k);
-    plan.oz_const = add_table(&k, sizeof(k));
-    plan.oz_moduli = nmod;
+hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
+  const char *tb = reinterpret_cast<const char *>(plan.d_tables);
+  const OzakiConsts *kc = reinterpret_cast<const OzakiConsts *>(tb + plan.oz_const);
+  switch (st.kind) {
+    case STEP_EW:
+      return launch_ew(st.op, reinterpret_cast<const EwPtrs *>(tb + st.table), st.count, st.n,
+                       st.vec2 != 0, s);
+    case STEP_GEMM_F64:
+      return launch_dgemm_batched(reinterpret_cast<const GemmPtrs *>(tb + st.table), st.count,
+                                  (int)st.n, s);
+    case STEP_REORDER:
+      return launch_leaf_reorder(reinterpret_cast<const ReorderPtrs *>(tb + st.table), st.count,
+                                 (int)st.n, st.bs, st.op == 1, s);
+    case STEP_OZ_GEMM:
+      return launch_residue_gemm(reinterpret_cast<const RGemm *>(tb + st.table), st.count,
+                                 (int)st.n, st.op, kc, s);
+    case STEP_OZ_SCALE:
+    case STEP_OZ_SPLIT:
+    case STEP_OZ_RECON:
+      return launch_ozaki_step(st.kind - STEP_OZ_SCALE,
+                               reinterpret_cast<const OzLeaf *>(tb + st.table), st.count,
+                               (int)st.n, st.op, kc, s);
+    case STEP_REW:
+      return launch_residue_ew(st.op, reinterpret_cast<const RPtrs *>(tb + st.table), st.count,
+                               st.n, plan.oz_moduli, st.vec2 != 0, kc, s);
+    case STEP_CRT_SCALE:
+    case STEP_CRT_SPLIT:
+    case STEP_CRT_RECON: {
+      CrtMats h;
+      memcpy(&h, plan.crt_host.data(), sizeof(h));
+      return launch_crt_step(st.kind - STEP_CRT_SCALE,
+                             reinterpret_cast<const CrtMats *>(tb + st.table), h, kc, s);
+    }
   }
-  const size_t ex_bytes = (size_t)m * sizeof(int);
-  const size_t pa_bytes = (size_t)OZ_CHUNKS * m * 2 * sizeof(double);
-  const size_t pb_bytes = (size_t)m * 2 * sizeof(double);
-  const size_t mark = top;
-  size_t EA = alloc(ex_bytes * nb), EB = alloc(ex_bytes * nb);
-  size_t PA = alloc(pa_bytes * nb), PB = alloc(pb_bytes * nb);
-  size_t RA = alloc(plane * nmod * nb), RB = alloc(plane * nmod * nb);
-  size_t RC = alloc(plane * nmod * nb);
-  std::vector<OzLeaf> t(nb);
-  for (size_t p = 0; p < nb; p++) {
-    OzLeaf &L = t[p];
-    L.A = work[p].A;
-    L.B = work[p].B;
-    L.C = work[p].C;
-    L.ea = reinterpret_cast<int *>(base + EA + ex_bytes * p);
-    L.eb = reinterpret_cast<int *>(base + EB + ex_bytes * p);
-    L.pa = reinterpret_cast<double *>(base + PA + pa_bytes * p);
-    L.pb = reinterpret_cast<double *>(base + PB + pb_bytes * p);
-    L.ra = reinterpret_cast<int8_t *>(base + RA + plane * nmod * p);
-    L.rb = reinterpret_cast<int8_t *>(base + RB + plane * nmod * p);
-    L.rc = reinterpret_cast<int8_t *>(base + RC + plane * nmod * p);
-  }
-  const size_t tab = add_table(t.data(), t.size() * sizeof(OzLeaf));
-  for (int kind : {STEP_OZ_SCALE, STEP_OZ_SPLIT, STEP_OZ_GEMM, STEP_OZ_RECON}) {
-    Step st{};
-    st.kind = kind;
-    st.count = (int)nb;
-    st.n = m;
-    st.op = nmod;
-    st.table = tab;
-    plan.steps.push_back(st);
-  }
-  top = mark;
+  return hipErrorInvalidValue;
 }
+
+int timer_of(int kind) {
+  switch (kind) {
+    case STEP_EW:
+    case STEP_REW: return CAPS_TIMER_ADD;
+    case STEP_GEMM_F64:
+    case STEP_OZ_GEMM: return CAPS_TIMER_MUL;
+    case STEP_REORDER: return CAPS_TIMER_REARRANGE;
+    case STEP_OZ_SCALE:
+    case STEP_OZ_SPLIT:
+    case STEP_CRT_SCALE:
+    case STEP_CRT_SPLIT: return CAPS_TIMER_SPLIT;
+    case STEP_OZ_RECON:
+    case STEP_CRT_RECON: return CAPS_TIMER_RECON;
+  }
+  return CAPS_TIMER_COMM;
+}
+
+}  // namespace
 
 int build_plan(Plan &plan, const double *A, const double *B, double *C, const caps_desc &d,
                const std::string &pattern, int mode, int moduli, void *work_base) {
@@ -318,14 +444,23 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
   plan.additions = plan.multiplications = plan.leaves = 0;
   plan.exec_pattern.clear();
   plan.oz_moduli = 0;
+  plan.crt_host.clear();
   Builder b{plan, reinterpret_cast<char *>(work_base)};
   b.bs = d.bs;
   b.mode = mode;
   b.moduli = moduli;
   b.pattern = pattern;
-  b.top_nrec = d.nrec;
-  std::vector<Prob> top{{A, B, C}};
-  b.rec(top, d.lda, d.nrec, 0);
+  b.residue = (mode == CAPS_MODE_CRT_INT8);
+  b.es = b.residue ? 1 : 8;
+  b.planes = b.residue ? moduli : 1;
+  if (b.residue) {
+    b.crt_top(A, B, C, d);
+  } else {
+    std::vector<Prob> top{{{reinterpret_cast<char *>(const_cast<double *>(A)), 0},
+                           {reinterpret_cast<char *>(const_cast<double *>(B)), 0},
+                           {reinterpret_cast<char *>(C), 0}}};
+    b.rec(top, d.lda, d.nrec, 0);
+  }
   plan.work_bytes = b.peak;
   return CAPS_OK;
 }
@@ -336,56 +471,6 @@ size_t plan_workspace_bytes(const caps_desc &d, const std::string &pattern, int 
   build_plan(p, nullptr, nullptr, nullptr, d, pattern, mode, moduli, nullptr);
   return p.work_bytes;
 }
-
-namespace {
-
-hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
-  const char *tb = reinterpret_cast<const char *>(plan.d_tables);
-  {
-    hipError_t e = hipSuccess;
-    switch (st.kind) {
-      case STEP_EW:
-        e = launch_ew(st.op, reinterpret_cast<const EwPtrs *>(tb + st.table), st.count, st.n,
-                      st.vec2 != 0, s);
-        break;
-      case STEP_GEMM_F64:
-        e = launch_dgemm_batched(reinterpret_cast<const GemmPtrs *>(tb + st.table), st.count,
-                                 (int)st.n, s);
-        break;
-      case STEP_REORDER:
-        e = launch_leaf_reorder(reinterpret_cast<const ReorderPtrs *>(tb + st.table), st.count,
-                                (int)st.n, st.bs, st.op == 1, s);
-        break;
-      case STEP_OZ_SCALE:
-      case STEP_OZ_SPLIT:
-      case STEP_OZ_GEMM:
-      case STEP_OZ_RECON:
-        e = launch_ozaki_step(st.kind - STEP_OZ_SCALE,
-                              reinterpret_cast<const OzLeaf *>(tb + st.table), st.count,
-                              (int)st.n, st.op,
-                              reinterpret_cast<const OzakiConsts *>(tb + plan.oz_const), s);
-        break;
-      default:
-        e = hipErrorInvalidValue;
-    }
-    return e;
-  }
-}
-
-int timer_of(int kind) {
-  switch (kind) {
-    case STEP_EW: return CAPS_TIMER_ADD;
-    case STEP_GEMM_F64:
-    case STEP_OZ_GEMM: return CAPS_TIMER_MUL;
-    case STEP_REORDER: return CAPS_TIMER_REARRANGE;
-    case STEP_OZ_SCALE:
-    case STEP_OZ_SPLIT: return CAPS_TIMER_SPLIT;
-    case STEP_OZ_RECON: return CAPS_TIMER_RECON;
-  }
-  return CAPS_TIMER_COMM;
-}
-
-}  // namespace
 
 hipError_t run_steps(const Plan &plan, hipStream_t s) {
   for (const Step &st : plan.steps) {

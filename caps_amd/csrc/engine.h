@@ -12,11 +12,16 @@
 
 namespace caps {
 
+// A matrix operand of a batched level: FP64 (plane stride unused) or nmod
+// INT8 residue planes, plane l at p + l * ps bytes.
+struct Mat {
+  char *p;
+  long long ps;
+};
+
 // one sub-problem of a batched level: C = A * B, all s x s in CAPS layout
 struct Prob {
-  const double *A;
-  const double *B;
-  double *C;
+  Mat A, B, C;
 };
 
 enum StepKind : int {
@@ -27,6 +32,10 @@ enum StepKind : int {
   STEP_OZ_SPLIT = 4, // split mode: scaled operands -> INT8 residue planes
   STEP_OZ_GEMM = 5,  // split mode: INT8 GEMM per modulus, residue outputs
   STEP_OZ_RECON = 6, // split mode: CRT reconstruction + unscaling
+  STEP_REW = 7,      // residue mode: Winograd adds on residue planes (EwOp in `op`)
+  STEP_CRT_SCALE = 8,  // residue mode: scale exponents of the whole A and B
+  STEP_CRT_SPLIT = 9,  // residue mode: A, B -> residue planes
+  STEP_CRT_RECON = 10, // residue mode: C residues -> C
 };
 
 struct Step {
@@ -63,6 +72,9 @@ struct Plan {
   // split-mode constants (moduli, CRT weights), device copy in d_tables
   size_t oz_const = 0;
   int oz_moduli = 0;
+  // residue mode: host copy of the top-level residue description
+  size_t crt_mats = 0;
+  std::vector<unsigned char> crt_host;
 };
 
 // Build a plan: pattern must already be resolved to one char per level.

@@ -16,8 +16,8 @@
 //     accumulation, reduced mod m_l in the epilogue (INT8 output plane);
 //  4. reconstruct: X = sum_l C'_l W_l  (mod M) by the Chinese remainder
 //     theorem in FP64 with the weights W_l split into 37-bit pieces, so every
-//     partial sum is exact; X = A'B' exactly, returned correctly rounded to
-//     FP64, then C = 2^-(sa_i + sb_j) X.
+//     partial sum is exact; X = A'B' exactly (assembled in 128 bits), returned
+//     correctly rounded to FP64, then C = 2^-(sa_i + sb_j) X.
 // Integer-valued leaves whose products fit 53 bits come out exact.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -25,8 +25,8 @@
 
 namespace caps {
 
-constexpr int OZ_MAX_MODULI = 20;
-constexpr int OZ_MAX_PIECES = 5;
+constexpr int OZ_MAX_MODULI = 16;  // M <= 2^128: X fits 128 bits
+constexpr int OZ_MAX_PIECES = 4;
 constexpr int OZ_PIECE_BITS = 37;
 constexpr int OZ_DEFAULT_MODULI = 14;
 // row / column chunks of the deterministic sum-of-squares reduction
@@ -57,6 +57,16 @@ struct OzLeaf {
   double *pb;       // m column sums of squares (B)
 };
 
+// One leaf product of the INT8 GEMM step: nmod residue planes per operand,
+// plane l of an operand at base + l * stride (bytes).  a: row-major m x m
+// (k contiguous), b and c: column-major m x m.
+struct RGemm {
+  const int8_t *a;
+  const int8_t *b;
+  int8_t *c;
+  long long sa, sb, sc;
+};
+
 // Host: moduli and CRT constants for nmod moduli (2..OZ_MAX_MODULI).
 // Returns false if nmod is out of range.
 bool ozaki_make_consts(int nmod, OzakiConsts &k);
@@ -64,8 +74,16 @@ bool ozaki_make_consts(int nmod, OzakiConsts &k);
 // The moduli used (greedy pairwise-coprime, descending from 256).
 const int *ozaki_moduli_table();
 
-// step: 0 = scale, 1 = split, 2 = INT8 GEMM, 3 = reconstruct
+// Number of 37-bit pieces of M for nmod moduli.
+int ozaki_npieces(int nmod);
+
+// step: 0 = scale, 1 = split, 3 = reconstruct (OzLeaf tables)
 hipError_t launch_ozaki_step(int step, const OzLeaf *d_tab, int count, int m, int nmod,
                              const OzakiConsts *d_consts, hipStream_t s);
+
+// C_l = A_l B_l (mod m_l) for every leaf of the table and every modulus, one
+// persistent launch.
+hipError_t launch_residue_gemm(const RGemm *d_tab, int count, int m, int nmod,
+                               const OzakiConsts *d_consts, hipStream_t s);
 
 }  // namespace caps

@@ -13,60 +13,12 @@
 
 #include "kernels.h"
 #include "ozaki.h"
+#include "ozaki_dev.h"
 #include "tilemap.h"
 
 namespace caps {
 
 namespace {
-
-constexpr int NOSCALE = -100000;  // exponent of an all-zero row / column
-
-// running (exponent e, sum s) with sum of squares = s * 4^e
-__device__ __forceinline__ void sq_acc(double a, int &e, double &s) {
-  if (a == 0.0) return;
-  int ea = ilogb(a);
-  if (ea > e) {
-    s = (e == NOSCALE) ? 0.0 : ldexp(s, 2 * (e - ea));
-    e = ea;
-  }
-  double t = ldexp(a, -e);
-  s = fma(t, t, s);
-}
-
-__device__ __forceinline__ void sq_merge(int &e, double &s, int e2, double s2) {
-  if (e2 == NOSCALE) return;
-  if (e == NOSCALE) {
-    e = e2;
-    s = s2;
-    return;
-  }
-  if (e2 > e) {
-    s = ldexp(s, 2 * (e - e2)) + s2;
-    e = e2;
-  } else {
-    s = s + ldexp(s2, 2 * (e2 - e));
-  }
-}
-
-// largest integer t with 2^t * 2^e * sqrt(s) * (1 + 2^-20) <= L
-__device__ __forceinline__ int scale_exponent(int e, double s, double L) {
-  if (e == NOSCALE || s == 0.0) return 0;
-  const double nm = sqrt(s) * (1.0 + 0x1p-20);  // rounding of s covered for K < 2^30
-  int t = ilogb(L) - e - ilogb(nm);
-  if (ldexp(nm, e + t) > L) t -= 1;
-  return t;
-}
-
-// symmetric residue of the FP64 integer x (|x| < 2^60) modulo mi
-__device__ __forceinline__ int sym_residue(double x, int mi, double inv) {
-  const double q = rint(x * inv);
-  const double r = fma(-q, (double)mi, x);  // exact: |r| < m
-  int ri = (int)r;
-  const int hi = (mi - 1) >> 1, lo = -(mi >> 1);
-  ri = ri > hi ? ri - mi : ri;
-  ri = ri < lo ? ri + mi : ri;
-  return ri;
-}
 
 constexpr int T256 = 256;
 
@@ -234,16 +186,6 @@ typedef unsigned v4u __attribute__((ext_vector_type(4)));
 constexpr int IBM = 128, IBN = 128, IBK = 64;
 constexpr int ISTR = IBK + 16;  // LDS row stride (bytes): b128 reads of 16 lanes hit 64 banks
 
-// symmetric residue of an INT32 product sum (|x| < 2^31) modulo mi
-__device__ __forceinline__ int mod_i32(int x, int mi, float inv) {
-  int q = __float2int_rn((float)x * inv);
-  int r = x - q * mi;
-  const int hi = (mi - 1) >> 1, lo = -(mi >> 1);
-  r = r > hi ? r - mi : r;
-  r = r < lo ? r + mi : r;
-  return r;
-}
-
 // global -> registers: 128 rows of A (k contiguous) and 128 columns of B
 // (k contiguous), 64 k each, 16 bytes per access
 template <bool CHECK>
@@ -293,7 +235,7 @@ __device__ __forceinline__ void lstore_i8(int8_t *As, int8_t *Bs, int tid, const
 
 // problem p = leaf * nmod + l: C_l = A_l * B_l (mod m_l)
 template <bool CHECK>
-__global__ __launch_bounds__(T256) void oz_igemm_kernel(const OzLeaf *__restrict__ tab, int m,
+__global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict__ tab, int m,
                                                         int nmod,
                                                         const OzakiConsts *__restrict__ kc,
                                                         TileMap map) {
@@ -304,7 +246,6 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const OzLeaf *__restrict
   const int l16 = lane & 15, lk = lane >> 4;
   const int G = gridDim.x;
   const int nk = (m + IBK - 1) / IBK;
-  const long long plane = (long long)m * m;
 
   for (int round = 0; round * G < map.total; round++) {
     const int tile = round * G + xcd_tile(blockIdx.x, G);
@@ -312,9 +253,9 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const OzLeaf *__restrict
     int prob, ti, tj;
     tile_coords(map, tile, prob, ti, tj);
     const int leaf = prob / nmod, l = prob - leaf * nmod;
-    const int8_t *__restrict__ Ap = tab[leaf].ra + l * plane;  // row-major
-    const int8_t *__restrict__ Bp = tab[leaf].rb + l * plane;  // column-major
-    int8_t *__restrict__ Cp = tab[leaf].rc + l * plane;
+    const int8_t *__restrict__ Ap = tab[leaf].a + l * tab[leaf].sa;  // row-major
+    const int8_t *__restrict__ Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major
+    int8_t *__restrict__ Cp = tab[leaf].c + l * tab[leaf].sc;
     const int mi = kc->m[l];
     const float invf = 1.0f / (float)mi;
     const int row0 = ti * IBM, col0 = tj * IBN;
@@ -372,29 +313,6 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const OzLeaf *__restrict
 }
 
 // ------------------------------------------------------------------ reconstruct
-// S_p += r_l * W_l[p]: exact, every partial sum < 2^49
-template <int NP>
-__device__ __forceinline__ void crt_acc(double (&S)[NP], int r, int l,
-                                        const OzakiConsts *__restrict__ kc) {
-  const double rd = (double)r;
-#pragma unroll
-  for (int p = 0; p < NP; p++) S[p] = fma(rd, kc->w[l][p], S[p]);
-}
-
-// X = sum_p S_p 2^(37p) reduced to (-M/2, M/2), correctly rounded
-template <int NP>
-__device__ __forceinline__ double crt_finish(const double (&S)[NP],
-                                             const OzakiConsts *__restrict__ kc) {
-  double approx = S[NP - 1];
-#pragma unroll
-  for (int p = NP - 2; p >= 0; p--) approx = fma(approx, 0x1p37, S[p]);
-  const double k = rint(approx * kc->invM);
-  double X = fma(-k, kc->Mp[NP - 1], S[NP - 1]);  // exact
-#pragma unroll
-  for (int p = NP - 2; p >= 0; p--) X = fma(X, 0x1p37, fma(-k, kc->Mp[p], S[p]));
-  return X;
-}
-
 template <int NP, bool VEC>
 __global__ __launch_bounds__(T256) void oz_recon_kernel(const OzLeaf *__restrict__ tab, int m,
                                                         int nmod,
@@ -451,12 +369,27 @@ hipError_t launch_recon(const OzLeaf *tab, int count, int m, int nmod, const Oza
   return hipGetLastError();
 }
 
-int npieces_for(int nmod) {
-  // log2 M <= 8 * nmod; pieces of 37 bits
-  return (8 * nmod + OZ_PIECE_BITS - 1) / OZ_PIECE_BITS;
-}
 
 }  // namespace
+
+hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int nmod, const OzakiConsts *kc,
+                               hipStream_t s) {
+  if (count <= 0 || m <= 0) return hipSuccess;
+  if (nmod < 2 || nmod > OZ_MAX_MODULI) return hipErrorInvalidValue;
+  bool ok;
+  TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);
+  if (!ok) return hipErrorInvalidValue;
+  const int G = persistent_grid(2, device_cu_count(), map.total);
+  // the fast path reads 16-byte rows: m % 128 == 0 and 16-byte aligned planes
+  bool fast = (m % IBM == 0);
+  if (fast)
+    hipLaunchKernelGGL((oz_igemm_kernel<false>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
+                       map);
+  else
+    hipLaunchKernelGGL((oz_igemm_kernel<true>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
+                       map);
+  return hipGetLastError();
+}
 
 hipError_t launch_ozaki_step(int step, const OzLeaf *tab, int count, int m, int nmod,
                              const OzakiConsts *kc, hipStream_t s) {
@@ -492,26 +425,12 @@ hipError_t launch_ozaki_step(int step, const OzLeaf *tab, int count, int m, int 
         hipLaunchKernelGGL((oz_split_b_kernel<true>), gb, dim3(T256), 0, s, tab, m, nmod, kc);
       return hipGetLastError();
     }
-    case 2: {
-      bool ok;
-      TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);
-      if (!ok) return hipErrorInvalidValue;
-      const int G = persistent_grid(2, device_cu_count(), map.total);
-      if (m % IBM == 0)
-        hipLaunchKernelGGL((oz_igemm_kernel<false>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
-                           map);
-      else
-        hipLaunchKernelGGL((oz_igemm_kernel<true>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
-                           map);
-      return hipGetLastError();
-    }
     case 3: {
-      switch (npieces_for(nmod)) {
+      switch (ozaki_npieces(nmod)) {
         case 1:
         case 2: return launch_recon<2>(tab, count, m, nmod, kc, s);
         case 3: return launch_recon<3>(tab, count, m, nmod, kc, s);
-        case 4: return launch_recon<4>(tab, count, m, nmod, kc, s);
-        default: return launch_recon<5>(tab, count, m, nmod, kc, s);
+        default: return launch_recon<4>(tab, count, m, nmod, kc, s);
       }
     }
   }

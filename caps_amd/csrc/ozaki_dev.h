@@ -1,0 +1,154 @@
+// Device helpers of the split-precision modes: deterministic norm bounds,
+// power-of-two scale exponents, symmetric residues and the CRT.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ozaki.h"
+
+namespace caps {
+
+constexpr int NOSCALE = -100000;  // exponent of an all-zero row / column
+
+// running (exponent e, sum s) with sum of squares = s * 4^e
+__device__ __forceinline__ void sq_acc(double a, int &e, double &s) {
+  if (a == 0.0) return;
+  int ea = ilogb(a);
+  if (ea > e) {
+    s = (e == NOSCALE) ? 0.0 : ldexp(s, 2 * (e - ea));
+    e = ea;
+  }
+  double t = ldexp(a, -e);
+  s = fma(t, t, s);
+}
+
+__device__ __forceinline__ void sq_merge(int &e, double &s, int e2, double s2) {
+  if (e2 == NOSCALE) return;
+  if (e == NOSCALE) {
+    e = e2;
+    s = s2;
+    return;
+  }
+  if (e2 > e) {
+    s = ldexp(s, 2 * (e - e2)) + s2;
+    e = e2;
+  } else {
+    s = s + ldexp(s2, 2 * (e2 - e));
+  }
+}
+
+// largest integer t with 2^t * 2^e * sqrt(s) * (1 + 2^-20) <= L
+__device__ __forceinline__ int scale_exponent(int e, double s, double L) {
+  if (e == NOSCALE || s == 0.0) return 0;
+  const double nm = sqrt(s) * (1.0 + 0x1p-20);  // rounding of s covered for K < 2^30
+  int t = ilogb(L) - e - ilogb(nm);
+  if (ldexp(nm, e + t) > L) t -= 1;
+  return t;
+}
+
+// symmetric residue of the FP64 integer x (|x| < 2^60) modulo mi
+__device__ __forceinline__ int sym_residue(double x, int mi, double inv) {
+  const double q = rint(x * inv);
+  const double r = fma(-q, (double)mi, x);  // exact: |r| < m
+  int ri = (int)r;
+  const int hi = (mi - 1) >> 1, lo = -(mi >> 1);
+  ri = ri > hi ? ri - mi : ri;
+  ri = ri < lo ? ri + mi : ri;
+  return ri;
+}
+
+// symmetric residue of an INT32 product sum (|x| < 2^31) modulo mi
+__device__ __forceinline__ int mod_i32(int x, int mi, float inv) {
+  int q = __float2int_rn((float)x * inv);
+  int r = x - q * mi;
+  const int hi = (mi - 1) >> 1, lo = -(mi >> 1);
+  r = r > hi ? r - mi : r;
+  r = r < lo ? r + mi : r;
+  return r;
+}
+
+// S_p += r_l * W_l[p]: exact, every partial sum < 2^49
+template <int NP>
+__device__ __forceinline__ void crt_acc(double (&S)[NP], int r, int l,
+                                        const OzakiConsts *__restrict__ kc) {
+  const double rd = (double)r;
+#pragma unroll
+  for (int p = 0; p < NP; p++) S[p] = fma(rd, kc->w[l][p], S[p]);
+}
+
+// correctly rounded conversion of a signed 128-bit integer to FP64
+__device__ __forceinline__ double i128_to_double(__int128 x) {
+  const bool neg = x < 0;
+  const unsigned __int128 a = neg ? (unsigned __int128)0 - (unsigned __int128)x : (unsigned __int128)x;
+  const unsigned long long hi = (unsigned long long)(a >> 64), lo = (unsigned long long)a;
+  const int bits = hi ? 128 - __clzll(hi) : (lo ? 64 - __clzll(lo) : 0);
+  double r;
+  if (bits <= 53) {
+    r = (double)lo;  // exact
+  } else {
+    const int sh = bits - 53;
+    unsigned __int128 q = a >> sh;
+    const unsigned __int128 rem = a - (q << sh);
+    const unsigned __int128 half = (unsigned __int128)1 << (sh - 1);
+    if (rem > half || (rem == half && (q & 1))) q += 1;  // round half to even
+    r = ldexp((double)(unsigned long long)q, sh);      // q <= 2^53: exact
+  }
+  return neg ? -r : r;
+}
+
+// X = sum_p S_p 2^(37p) reduced to (-M/2, M/2): the quotient k = round(S/M)
+// from an FP64 estimate (exact unless X were within 2^-40 M of M/2, which the
+// scaling bound excludes), D_p = S_p - k M_p exact, X assembled exactly in
+// 128 bits (|X| < M/2 <= 2^127) and rounded once to FP64
+template <int NP>
+__device__ __forceinline__ double crt_finish(const double (&S)[NP],
+                                             const OzakiConsts *__restrict__ kc) {
+  double approx = S[NP - 1];
+#pragma unroll
+  for (int p = NP - 2; p >= 0; p--) approx = fma(approx, 0x1p37, S[p]);
+  const double k = rint(approx * kc->invM);
+  __int128 X = 0;
+#pragma unroll
+  for (int p = NP - 1; p >= 0; p--)
+    X = X * ((__int128)1 << OZ_PIECE_BITS) + (__int128)(long long)fma(-k, kc->Mp[p], S[p]);
+  return i128_to_double(X);
+}
+
+// offset of element (i, j) of an n x n matrix in the CAPS layout (nproc = 1):
+// nrec quadrant levels in the order 11, 21, 12, 22, then bs x bs blocks
+// column-major between and within blocks (matrix.h:20-44)
+__device__ __forceinline__ long long caps_elem_offset(int n, int nrec, int bs, int i, int j) {
+  long long off = 0;
+  long long s = n;
+  for (int l = 0; l < nrec; l++) {
+    long long h = s >> 1;
+    int bi = i >= h, bj = j >= h;
+    off += (long long)(bi + 2 * bj) * h * h;
+    i -= bi * (int)h;
+    j -= bj * (int)h;
+    s = h;
+  }
+  long long nb = s / bs;
+  return off + ((long long)(i / bs) + (long long)(j / bs) * nb) * bs * bs + (i % bs) +
+         (long long)(j % bs) * bs;
+}
+
+// offset of (li, lj) inside an m x m leaf stored as bs x bs blocks
+__device__ __forceinline__ long long leaf_elem_offset(int m, int bs, int li, int lj) {
+  const long long nb = m / bs;
+  return ((long long)(li / bs) + (long long)(lj / bs) * nb) * bs * bs + (li % bs) +
+         (long long)(lj % bs) * bs;
+}
+
+// leaf index L (CAPS order: base-4 digits, quadrant 11, 21, 12, 22 per level)
+// -> leaf row block bi and leaf column block bj
+__device__ __forceinline__ void leaf_coords(int L, int nrec, int &bi, int &bj) {
+  bi = 0;
+  bj = 0;
+  for (int l = 0; l < nrec; l++) {
+    const int q = (L >> (2 * (nrec - 1 - l))) & 3;
+    bi = 2 * bi + (q & 1);
+    bj = 2 * bj + (q >> 1);
+  }
+}
+
+}  // namespace caps

@@ -109,6 +109,13 @@ const int *ozaki_moduli_table() {
   return t.data();
 }
 
+int ozaki_npieces(int nmod) {
+  if (nmod < 2 || nmod > OZ_MAX_MODULI) return OZ_MAX_PIECES;
+  Big M(1);
+  for (int l = 0; l < nmod; l++) M.mul((uint32_t)ozaki_moduli_table()[l]);
+  return (M.bitlen() + OZ_PIECE_BITS - 1) / OZ_PIECE_BITS;
+}
+
 bool ozaki_make_consts(int nmod, OzakiConsts &k) {
   if (nmod < 2 || nmod > OZ_MAX_MODULI) return false;
   memset(&k, 0, sizeof(k));

@@ -56,16 +56,24 @@ typedef enum caps_status {
 /* Leaf (base-case) product arithmetic. */
 typedef enum caps_mode {
   CAPS_MODE_FP64 = 0,       /* FP64 MFMA leaf GEMM                           */
-  CAPS_MODE_OZAKI_INT8 = 1, /* FP64 operands as scaled integers, split into
-                               residues modulo pairwise-coprime moduli <= 256
-                               (INT8), exact INT8 MFMA products with INT32
-                               accumulation, CRT reconstruction in FP64      */
+  CAPS_MODE_OZAKI_INT8 = 1, /* leaf-only split: FP64 Winograd additions; each
+                               leaf product from FP64 operands scaled to
+                               integers, split into residues modulo
+                               pairwise-coprime moduli <= 256 (INT8), exact
+                               INT8 MFMA products with INT32 accumulation and
+                               CRT reconstruction in FP64                    */
+  CAPS_MODE_CRT_INT8 = 2,   /* residue-domain Strassen: A and B scaled and
+                               split into INT8 residues once, every Winograd
+                               addition exact modulo each m_l, INT8 MFMA leaf
+                               products, one CRT reconstruction of C; exact
+                               for integer products below 2^53, otherwise
+                               only the initial scaling truncates (nrec <= 7) */
   CAPS_MODE_AUTO = 99       /* the planner (autotuner / learned model) picks */
 } caps_mode;
 
 typedef struct caps_opts {
   int mode;      /* caps_mode                                              */
-  int moduli;    /* split modes: number of moduli (0 = default 14, 2..20)  */
+  int moduli;    /* split modes: number of moduli (0 = default 14, 2..16)  */
   int flags;     /* CAPS_FLAG_* below                                      */
   int reserved;
 } caps_opts;

@@ -1,0 +1,121 @@
+"""GPU parity of the two split-precision modes through the C ABI.
+
+  * CAPS_MODE_CRT_INT8 (residue-domain Strassen): bit-exact against the
+    exact-integer model oracle/split.py on real inputs, for every pattern;
+    bit-exact on integers (fixtures, [-64, 64]); normwise error <= 1e-14
+    against the compensated (Dot2) product on u * 2^e inputs.
+  * CAPS_MODE_OZAKI_INT8 (FP64 Winograd additions, split leaves): bit-exact
+    against the model's FP64 Winograd with scaled exact leaves.
+"""
+import numpy as np
+import pytest
+
+import caps_amd
+import oracle
+from oracle import split
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+CRT = caps_amd.MODE_CRT_INT8
+LEAF = caps_amd.MODE_OZAKI_INT8
+
+
+def run(a, b, n, r, bs=1, pattern=None, mode=CRT, moduli=0, flags=0):
+    print(f"[gpu] multiply n={n} r={r} bs={bs} pattern={pattern} mode={mode} "
+          f"moduli={moduli} flags={flags}", flush=True)
+    d = caps_amd.MatDescriptor.one_device(n, r, bs)
+    dA = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    dB = torch.from_numpy(np.ascontiguousarray(b)).cuda()
+    dC = torch.empty_like(dA)
+    caps_amd.multiply_device(dA, dB, dC, d, pattern, mode, moduli, flags)
+    torch.cuda.synchronize()
+    return dC.cpu().numpy()
+
+
+def realdata(n, seed, spread=10):
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-spread, spread + 1, (n, n)))
+    B = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-spread, spread + 1, (n, n)))
+    return A, B
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _setup():
+    oracle.build()
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    caps_amd.load()
+    yield
+    caps_amd.release()
+
+
+@pytest.mark.parametrize("n,r,bs,patterns,moduli", [
+    (64, 2, 1, ["bb", "dd", "bd", "db"], 14),
+    (96, 1, 1, ["b", "d"], 14),
+    (128, 2, 8, ["bb", "dh"], 16),     # bs-blocked leaves
+    (56, 3, 1, ["bbb", "ddd"], 12),    # odd leaves (m = 7)
+    (256, 1, 1, ["b"], 14),            # 128 x 128 leaves: the fast GEMM path
+])
+def test_crt_mode_bit_exact_vs_model(n, r, bs, patterns, moduli):
+    A, B = realdata(n, 100 + n)
+    want = split.crt_mode(A, B, moduli)
+    a, b = oracle.to_caps(A, r, bs), oracle.to_caps(B, r, bs)
+    for pat in patterns:
+        got = oracle.from_caps(run(a, b, n, r, bs, pat, CRT, moduli), n, r, bs)
+        assert np.array_equal(got, want), (pat, np.abs(got - want).max())
+
+
+def test_leaf_split_mode_bit_exact_vs_model():
+    n, r = 64, 2
+    A, B = realdata(n, 7)
+    want = split.leaf_split_mode(A, B, r, 14)
+    for pat in ("bb", "dd"):
+        got = oracle.from_caps(run(oracle.to_caps(A, r, 1), oracle.to_caps(B, r, 1), n, r, 1,
+                                   pat, LEAF, 14), n, r, 1)
+        assert np.array_equal(got, want), pat
+
+
+@pytest.mark.parametrize("size", oracle.FIXTURE_SIZES)
+def test_crt_mode_fixtures_exact(size):
+    A, B, C = oracle.read_fixture(size)
+    for nproc, bs, r in oracle.read_params(size):
+        k = {1: 0, 7: 1, 49: 2, 343: 3}[nproc]
+        pat = "b" * k + "d" * (r - k)
+        c = run(oracle.to_caps(A, r, bs), oracle.to_caps(B, r, bs), size, r, bs, pat, CRT)
+        assert np.array_equal(oracle.from_caps(c, size, r, bs), C), (bs, r, pat)
+
+
+def test_crt_mode_integers_exact_bb():
+    n, r = 1024, 2
+    rng = np.random.default_rng(64)
+    A = rng.integers(-64, 65, (n, n)).astype(np.float64)
+    B = rng.integers(-64, 65, (n, n)).astype(np.float64)
+    c = run(oracle.to_caps(A, r, 1), oracle.to_caps(B, r, 1), n, r, 1, "bb", CRT)
+    assert np.array_equal(oracle.from_caps(c, n, r, 1), oracle.igemm_exact(A, B))
+
+
+@pytest.mark.parametrize("n,r,pattern", [(1024, 2, "bb"), (1024, 3, "bdd"), (768, 1, "d")])
+def test_crt_mode_error_below_bar(n, r, pattern):
+    """u * 2^e inputs, e in [-10, 10]: normwise error <= 1e-14 (the metric's
+    bar) -- and below the FP64 Winograd floor of the same pattern."""
+    A, B = realdata(n, n + r)
+    a, b = oracle.to_caps(A, r, 1), oracle.to_caps(B, r, 1)
+    ref = oracle.dgemm_dot2(A, B)
+    got = oracle.from_caps(run(a, b, n, r, 1, pattern, CRT, 14), n, r, 1)
+    err = float(np.abs(got - ref).max() / np.abs(ref).max())
+    fp64 = oracle.from_caps(oracle.multiply(a, b, n, r, 1, pattern, 8), n, r, 1)
+    err_w = float(np.abs(fp64 - ref).max() / np.abs(ref).max())
+    print(f"crt {n} {pattern}: error {err:.3e}, FP64 Winograd {err_w:.3e}")
+    assert err <= 1e-14
+    assert err < err_w
+
+
+def test_crt_mode_pattern_independent():
+    n, r = 512, 2
+    A, B = realdata(n, 77)
+    a, b = oracle.to_caps(A, r, 1), oracle.to_caps(B, r, 1)
+    ref = run(a, b, n, r, 1, "bb", CRT)
+    for pat in ("dd", "bd", "db", "hb"):
+        assert np.array_equal(run(a, b, n, r, 1, pat, CRT), ref), pat
