@@ -55,9 +55,29 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_drivers()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources")
+        print(f"built {LIB} from {len(srcs)} sources, drivers in {BIN}")
     return LIB
+
+
+BIN = os.path.join(HERE, "bin")
+DRIVERS = os.path.join(HERE, "drivers")
+
+
+def build_drivers() -> None:
+    """The C++ drivers use only include/caps/*.h and the C ABI: plain g++."""
+    os.makedirs(BIN, exist_ok=True)
+    for f in sorted(os.listdir(DRIVERS)):
+        if not f.endswith(".cpp"):
+            continue
+        out = os.path.join(BIN, f[:-4])
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+               os.path.join(DRIVERS, f), "-o", out, "-L", HERE, "-lcaps_amd",
+               "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed for {f}:\n{r.stderr}")
 
 
 if __name__ == "__main__":
