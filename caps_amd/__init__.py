@@ -99,6 +99,15 @@ SIGNATURES = {
     "caps_reset_counters": (ctypes.c_int, []),
     "caps_set_available_memory": (ctypes.c_int, [ctypes.c_longlong]),
     "caps_release": (ctypes.c_int, []),
+    "caps_mlp_nparams": (ctypes.c_int, []),
+    "caps_mlp_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_ulonglong]),
+    "caps_mlp_destroy": (ctypes.c_int, [_vp]),
+    "caps_mlp_get_params": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
+    "caps_mlp_set_params": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
+    "caps_mlp_forward": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    "caps_mlp_gradient": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
+    "caps_mlp_train": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong, _vp]),
 }
 
 _lib = None
@@ -269,3 +278,63 @@ def print_counters(t: float, desc: MatDescriptor) -> str:
     text = "\n".join(lines)
     print(text)
     return text
+
+
+# ------------------------------------------------------------------ cost / error model
+class MLP:
+    """The planner's 3-layer perceptron (8 -> 64 -> 64 -> 2, ReLU, FP32) whose
+    forward, backward and Adam kernels run on the GPU (caps_mlp_* in the C ABI)."""
+
+    N_IN, N_OUT = 8, 2
+
+    def __init__(self, seed: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(load().caps_mlp_create(ctypes.byref(self._h), seed))
+        self.nparams = load().caps_mlp_nparams()
+
+    def close(self):
+        if self._h:
+            load().caps_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _f32(a, cols):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.ndim == 2 and a.shape[1] == cols
+        return a
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, dtype=np.float32)
+        _check(load().caps_mlp_get_params(self._h, out.ctypes.data, self.nparams))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(load().caps_mlp_set_params(self._h, p.ctypes.data, self.nparams))
+
+    def forward(self, X) -> np.ndarray:
+        X = self._f32(X, self.N_IN)
+        Y = np.empty((X.shape[0], self.N_OUT), dtype=np.float32)
+        _check(load().caps_mlp_forward(self._h, X.ctypes.data, X.shape[0], Y.ctypes.data))
+        return Y
+
+    def gradient(self, X, T, M):
+        X, T, M = self._f32(X, self.N_IN), self._f32(T, self.N_OUT), self._f32(M, self.N_OUT)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(load().caps_mlp_gradient(self._h, X.ctypes.data, T.ctypes.data, M.ctypes.data,
+                                        X.shape[0], g.ctypes.data, ctypes.byref(loss)))
+        return g, float(loss.value)
+
+    def train(self, X, T, M, epochs=200, batch=256, lr=1e-3, seed=0) -> float:
+        X, T, M = self._f32(X, self.N_IN), self._f32(T, self.N_OUT), self._f32(M, self.N_OUT)
+        loss = ctypes.c_float()
+        _check(load().caps_mlp_train(self._h, X.ctypes.data, T.ctypes.data, M.ctypes.data,
+                                     X.shape[0], epochs, batch, lr, seed, ctypes.byref(loss)))
+        return float(loss.value)

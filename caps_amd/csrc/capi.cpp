@@ -286,6 +286,8 @@ int stage_locked(State &s, size_t elems) {
 
 int device_cu_count() { return st().cus; }
 
+int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+
 }  // namespace caps
 
 using namespace caps;

@@ -165,6 +165,36 @@ int caps_set_available_memory(long long doubles);
 /* Release cached plans, graphs and workspace. */
 int caps_release(void);
 
+/* ------------------------------------------------------------------------
+ * Planner cost / error model (no counterpart in the reference, which picks
+ * BFS or DFS from a memory rule, multiply.cpp:43-66): a 3-layer perceptron
+ * 8 -> 64 -> 64 -> 2 (ReLU, FP32) whose forward, backward and Adam steps run
+ * as HIP kernels.  Inputs and targets are the caller's normalised features
+ * (row-major B x 8) and targets (B x 2) with a 0/1 mask per target.  The
+ * parameter vector is W1 (64x8), b1, W2 (64x64), b2, W3 (2x64), b3.
+ * ------------------------------------------------------------------------ */
+typedef struct caps_mlp caps_mlp;
+
+/* number of parameters (not a status) */
+int caps_mlp_nparams(void);
+/* He-uniform weights from a splitmix64 stream seeded with `seed`, zero biases */
+int caps_mlp_create(caps_mlp **out, unsigned long long seed);
+int caps_mlp_destroy(caps_mlp *m);
+int caps_mlp_get_params(caps_mlp *m, float *out, int n);
+/* also resets the optimiser state */
+int caps_mlp_set_params(caps_mlp *m, const float *in, int n);
+/* Y (B x 2) = model(X (B x 8)), host arrays */
+int caps_mlp_forward(caps_mlp *m, const float *X, int B, float *Y);
+/* gradient of mean_s sum_k M (Y - T)^2 with respect to the parameters, and
+ * the loss itself (loss may be NULL) */
+int caps_mlp_gradient(caps_mlp *m, const float *X, const float *T, const float *M, int B,
+                      float *grad, float *loss);
+/* Adam (lr, betas 0.9 / 0.999, eps 1e-8) over `epochs` seeded shuffles in
+ * minibatches of `batch`; final_loss (may be NULL) over all B samples */
+int caps_mlp_train(caps_mlp *m, const float *X, const float *T, const float *M, int B,
+                   int epochs, int batch, float lr, unsigned long long seed,
+                   float *final_loss);
+
 #ifdef __cplusplus
 }
 #endif
