@@ -69,7 +69,7 @@ assert ctypes.sizeof(MatDescriptor) == DESC_SIZE * ctypes.sizeof(ctypes.c_int)
 
 class Options(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("moduli", ctypes.c_int), ("flags", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("cutoff", ctypes.c_int)]
 
 
 class Counters(ctypes.Structure):
@@ -138,8 +138,8 @@ def _check(rc: int):
         raise CapsError(rc, load().caps_last_error().decode(errors="replace"))
 
 
-def _opts(mode: int, moduli: int, flags: int) -> Options:
-    return Options(mode, moduli, flags, 0)
+def _opts(mode: int, moduli: int, flags: int, cutoff: int = 0) -> Options:
+    return Options(mode, moduli, flags, cutoff)
 
 
 def _pattern(p):
@@ -170,11 +170,11 @@ def _host_ptr(a: np.ndarray, n2: int, writable: bool = False):
 
 def multiply(A: np.ndarray, B: np.ndarray, C: np.ndarray, desc: MatDescriptor,
              pattern: str | None = None, mode: int = MODE_FP64, moduli: int = 0,
-             flags: int = 0) -> None:
+             flags: int = 0, cutoff: int = 0) -> None:
     """C = A * B on the device, all three in the CAPS layout (host memory).
     multiply.h:13: A and B are read, C is overwritten."""
     n2 = desc.lda * desc.lda
-    o = _opts(mode, moduli, flags)
+    o = _opts(mode, moduli, flags, cutoff)
     _check(load().caps_multiply(_host_ptr(A, n2), _host_ptr(B, n2), _host_ptr(C, n2, True),
                                 ctypes.byref(desc), _pattern(pattern), ctypes.byref(o)))
 
@@ -196,11 +196,11 @@ def _stream_ptr(stream):
 
 def multiply_device(A, B, C, desc: MatDescriptor, pattern: str | None = None,
                     mode: int = MODE_FP64, moduli: int = 0, flags: int = 0,
-                    stream=None) -> None:
+                    stream=None, cutoff: int = 0) -> None:
     """Enqueue C = A * B on `stream` (default: torch's current stream) for
     device-resident CAPS-layout tensors.  Does not synchronise."""
     n2 = desc.lda * desc.lda
-    o = _opts(mode, moduli, flags)
+    o = _opts(mode, moduli, flags, cutoff)
     _check(load().caps_multiply_device(_dev_ptr(A, n2), _dev_ptr(B, n2), _dev_ptr(C, n2),
                                        ctypes.byref(desc), _pattern(pattern), ctypes.byref(o),
                                        _stream_ptr(stream)))

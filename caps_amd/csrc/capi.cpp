@@ -158,9 +158,24 @@ int resolve_mode(const caps_desc &d, const caps_opts *o, int &mode, int &moduli)
   return CAPS_OK;
 }
 
+int check_cutoff(const caps_desc &d, int mode, int cutoff) {
+  if (cutoff < 0 || cutoff > d.nrec)
+    return fail(CAPS_EINVAL, "cutoff=%d outside 0..nrec=%d", cutoff, d.nrec);
+  if (cutoff == 0) return CAPS_OK;
+  const int m0 = d.lda >> d.nrec;
+  if (mode == CAPS_MODE_OZAKI_INT8)
+    return fail(CAPS_EUNSUPPORTED, "cutoff > 0 needs mode FP64 or CRT_INT8");
+  if (m0 % 128 != 0)
+    return fail(CAPS_EUNSUPPORTED, "cutoff > 0 needs leaves that are multiples of 128 (leaf %d)",
+                m0);
+  if (d.bs != 1 && d.bs != m0)
+    return fail(CAPS_EUNSUPPORTED, "cutoff > 0 needs bs = 1 or bs = leaf size");
+  return CAPS_OK;
+}
+
 int get_plan_locked(State &s, const double *A, const double *B, double *C, const caps_desc &d,
-                    const std::string &pat, int mode, int moduli, Plan *&out) {
-  PlanKey key{A, B, C, d, pat, mode, moduli};
+                    const std::string &pat, int mode, int moduli, int cutoff, Plan *&out) {
+  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff};
   for (auto &p : s.plans)
     if (p->key == key) {
       out = p.get();
@@ -170,7 +185,7 @@ int get_plan_locked(State &s, const double *A, const double *B, double *C, const
   evict_locked(s, MAX_PLANS - 1);
   auto plan = std::make_unique<Plan>();
   plan->key = key;
-  build_plan(*plan, A, B, C, d, pat, mode, moduli, nullptr);
+  build_plan(*plan, A, B, C, d, pat, mode, moduli, cutoff, nullptr);
   const size_t need = plan->work_bytes;
   if (need) {
     size_t freeb = 0, totalb = 0;
@@ -186,7 +201,7 @@ int get_plan_locked(State &s, const double *A, const double *B, double *C, const
     if (e != hipSuccess)
       return fail(CAPS_ENOMEM, "hipMalloc(%zu) for workspace: %s", need, hipGetErrorString(e));
   }
-  build_plan(*plan, A, B, C, d, pat, mode, moduli, plan->d_work);
+  build_plan(*plan, A, B, C, d, pat, mode, moduli, cutoff, plan->d_work);
   if (!plan->host_tables.empty()) {
     hipError_t e = hipMalloc(&plan->d_tables, plan->host_tables.size());
     if (e != hipSuccess) {
@@ -235,11 +250,14 @@ int multiply_device_locked(State &s, const double *dA, const double *dB, double 
   int mode, moduli;
   rc = resolve_mode(*desc, opts, mode, moduli);
   if (rc) return rc;
+  const int cutoff = opts ? opts->cutoff : 0;
+  rc = check_cutoff(*desc, mode, cutoff);
+  if (rc) return rc;
   std::string pat;
   rc = resolve_pattern(*desc, pattern, opts, pat, mode, moduli, s.available_doubles);
   if (rc) return rc;
   Plan *plan = nullptr;
-  rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, plan);
+  rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, cutoff, plan);
   if (rc) return rc;
   const bool timed = opts && (opts->flags & CAPS_FLAG_TIMERS);
   const bool use_graph = !timed && !(opts && (opts->flags & CAPS_FLAG_NO_GRAPH));

@@ -32,7 +32,7 @@ namespace caps {
 
 bool PlanKey::operator==(const PlanKey &o) const {
   return A == o.A && B == o.B && C == o.C && memcmp(&desc, &o.desc, sizeof(desc)) == 0 &&
-         pattern == o.pattern && mode == o.mode && moduli == o.moduli;
+         pattern == o.pattern && mode == o.mode && moduli == o.moduli && cutoff == o.cutoff;
 }
 
 namespace {
@@ -46,6 +46,7 @@ struct Builder {
   int bs = 1;
   int mode = CAPS_MODE_FP64;
   int moduli = 0;
+  int cutoff = 0;  // layout levels below the Strassen recursion
   std::string pattern;
   bool residue = false;  // matrices are nmod INT8 planes
   int es = 8;            // bytes per element: 8 (FP64) or 1 (residue)
@@ -84,8 +85,10 @@ struct Builder {
     }
   }
 
-  void push(int kind, int op, int count, long long n, int bs_, size_t table, int vec = 0) {
+  void push(int kind, int op, int count, long long n, int bs_, size_t table, int vec = 0,
+            int levels = 0) {
     Step st{};
+    st.levels = levels;
     st.kind = kind;
     st.op = op;
     st.count = count;
@@ -164,12 +167,13 @@ struct Builder {
                 reinterpret_cast<int8_t *>(batch[p].C.p), batch[p].A.ps, batch[p].B.ps,
                 batch[p].C.ps};
       push(STEP_OZ_GEMM, moduli, (int)g.size(), m, 0,
-           add_table(g.data(), g.size() * sizeof(RGemm)));
+           add_table(g.data(), g.size() * sizeof(RGemm)), 0, cutoff);
       return;
     }
     const size_t mark = top;
     std::vector<Prob> work = batch;
-    const bool reorder = (bs != 1 && bs != m);
+    // bs-blocked leaves of the layout (size m >> cutoff) need a reorder
+    const bool reorder = (bs != 1 && bs != (m >> cutoff));
     const long long mm = (long long)m * m;
     if (reorder) {
       Mat AA = alloc_mats(batch.size(), mm), BB = alloc_mats(batch.size(), mm),
@@ -193,7 +197,8 @@ struct Builder {
       for (auto &p : work)
         g.push_back({reinterpret_cast<const double *>(p.A.p),
                      reinterpret_cast<const double *>(p.B.p), reinterpret_cast<double *>(p.C.p)});
-      push(STEP_GEMM_F64, 0, (int)g.size(), m, 0, add_table(g.data(), g.size() * sizeof(GemmPtrs)));
+      push(STEP_GEMM_F64, 0, (int)g.size(), m, 0, add_table(g.data(), g.size() * sizeof(GemmPtrs)),
+           0, cutoff);
     } else {
       ozaki_leaf_steps(work, m);
     }
@@ -333,7 +338,7 @@ struct Builder {
   }
 
   void rec(const std::vector<Prob> &batch, int s, int nrec, int depth) {
-    if (nrec == 0) {
+    if (nrec == cutoff) {
       leaf(batch, s);
       return;
     }
@@ -390,13 +395,13 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
                        st.vec2 != 0, s);
     case STEP_GEMM_F64:
       return launch_dgemm_batched(reinterpret_cast<const GemmPtrs *>(tb + st.table), st.count,
-                                  (int)st.n, s);
+                                  (int)st.n, st.levels, s);
     case STEP_REORDER:
       return launch_leaf_reorder(reinterpret_cast<const ReorderPtrs *>(tb + st.table), st.count,
                                  (int)st.n, st.bs, st.op == 1, s);
     case STEP_OZ_GEMM:
       return launch_residue_gemm(reinterpret_cast<const RGemm *>(tb + st.table), st.count,
-                                 (int)st.n, st.op, kc, s);
+                                 (int)st.n, st.levels, st.op, kc, s);
     case STEP_OZ_SCALE:
     case STEP_OZ_SPLIT:
     case STEP_OZ_RECON:
@@ -438,7 +443,7 @@ int timer_of(int kind) {
 }  // namespace
 
 int build_plan(Plan &plan, const double *A, const double *B, double *C, const caps_desc &d,
-               const std::string &pattern, int mode, int moduli, void *work_base) {
+               const std::string &pattern, int mode, int moduli, int cutoff, void *work_base) {
   plan.steps.clear();
   plan.host_tables.clear();
   plan.additions = plan.multiplications = plan.leaves = 0;
@@ -449,6 +454,7 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
   b.bs = d.bs;
   b.mode = mode;
   b.moduli = moduli;
+  b.cutoff = cutoff;
   b.pattern = pattern;
   b.residue = (mode == CAPS_MODE_CRT_INT8);
   b.es = b.residue ? 1 : 8;
@@ -466,9 +472,9 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
 }
 
 size_t plan_workspace_bytes(const caps_desc &d, const std::string &pattern, int mode,
-                            int moduli) {
+                            int moduli, int cutoff) {
   Plan p;
-  build_plan(p, nullptr, nullptr, nullptr, d, pattern, mode, moduli, nullptr);
+  build_plan(p, nullptr, nullptr, nullptr, d, pattern, mode, moduli, cutoff, nullptr);
   return p.work_bytes;
 }
 

@@ -45,6 +45,7 @@ struct Step {
   long long n;    // elements per quadrant (EW) or leaf size m (GEMM/REORDER)
   int bs;
   int vec2;
+  int levels;     // leaf GEMMs: quadrant levels inside the operands (cutoff)
   size_t table;   // byte offset of this step's pointer table
   size_t aux;     // byte offset of an auxiliary workspace region
 };
@@ -53,7 +54,7 @@ struct PlanKey {
   const void *A, *B, *C;
   caps_desc desc;
   std::string pattern;
-  int mode, moduli;
+  int mode, moduli, cutoff;
   bool operator==(const PlanKey &o) const;
 };
 
@@ -81,7 +82,7 @@ struct Plan {
 // work_base = nullptr computes sizes only.
 int build_plan(Plan &plan, const double *A, const double *B, double *C,
                const caps_desc &d, const std::string &pattern, int mode, int moduli,
-               void *work_base);
+               int cutoff, void *work_base);
 
 // Enqueue every step of the plan on `s` (no graph).
 hipError_t run_steps(const Plan &plan, hipStream_t s);
@@ -93,6 +94,6 @@ hipError_t run_steps_timed(const Plan &plan, hipStream_t s, double *ms, long lon
 
 // Workspace (bytes) a resolved pattern needs.
 size_t plan_workspace_bytes(const caps_desc &d, const std::string &pattern, int mode,
-                            int moduli);
+                            int moduli, int cutoff = 0);
 
 }  // namespace caps

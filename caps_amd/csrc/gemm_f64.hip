@@ -31,31 +31,33 @@ constexpr int SA = BM + 16;  // As[k][row]
 constexpr int SB = BK + 2;   // Bs[col][k]
 constexpr int NT = 256;
 // global -> registers: the A slice (128 rows x 16 k) and B slice (16 k x 128
-// cols) of one K step, two doubles per access
+// cols) of one K step, two doubles per access.  Ab / Bb point at the slice's
+// first element, ld is the column stride (the leaf size).  CHECK: bounds
+// against m with the slice origin (row0, k0, col0); only for plain operands.
 template <bool CHECK>
-__device__ __forceinline__ void gload_f64(const double *__restrict__ A,
-                                          const double *__restrict__ B, int m, int row0,
-                                          int col0, int k0, int tid, v2d (&ra)[4],
+__device__ __forceinline__ void gload_f64(const double *__restrict__ Ab,
+                                          const double *__restrict__ Bb, long long ld, int m,
+                                          int row0, int col0, int k0, int tid, v2d (&ra)[4],
                                           v2d (&rb)[4]) {
 #pragma unroll
   for (int it = 0; it < 4; it++) {
     const int idx = tid + NT * it;
     // A: column kc of the slice, rows 2*r2, 2*r2+1
     const int kc = idx >> 6, r2 = idx & 63;
-    const int ar = row0 + 2 * r2, ak = k0 + kc;
     // B: column c of the tile, k rows 2*k2, 2*k2+1
     const int c = idx >> 3, k2 = idx & 7;
-    const int bk = k0 + 2 * k2, bc = col0 + c;
     if (!CHECK) {
-      ra[it] = *reinterpret_cast<const v2d *>(A + (long long)ak * m + ar);
-      rb[it] = *reinterpret_cast<const v2d *>(B + (long long)bc * m + bk);
+      ra[it] = *reinterpret_cast<const v2d *>(Ab + kc * ld + 2 * r2);
+      rb[it] = *reinterpret_cast<const v2d *>(Bb + c * ld + 2 * k2);
     } else {
+      const int ar = row0 + 2 * r2, ak = k0 + kc;
+      const int bk = k0 + 2 * k2, bc = col0 + c;
       const bool kok = ak < m;
-      ra[it].x = (kok && ar < m) ? A[(long long)ak * m + ar] : 0.0;
-      ra[it].y = (kok && ar + 1 < m) ? A[(long long)ak * m + ar + 1] : 0.0;
+      ra[it].x = (kok && ar < m) ? Ab[kc * ld + 2 * r2] : 0.0;
+      ra[it].y = (kok && ar + 1 < m) ? Ab[kc * ld + 2 * r2 + 1] : 0.0;
       const bool cok = bc < m;
-      rb[it].x = (cok && bk < m) ? B[(long long)bc * m + bk] : 0.0;
-      rb[it].y = (cok && bk + 1 < m) ? B[(long long)bc * m + bk + 1] : 0.0;
+      rb[it].x = (cok && bk < m) ? Bb[c * ld + 2 * k2] : 0.0;
+      rb[it].y = (cok && bk + 1 < m) ? Bb[c * ld + 2 * k2 + 1] : 0.0;
     }
   }
 }
@@ -73,9 +75,12 @@ __device__ __forceinline__ void lstore_f64(double *As, double *Bs, int tid,
   }
 }
 
+// Operands are m x m, stored as a quadtree of 2^d x 2^d column-major leaves
+// of size m0 = m >> d in the CAPS quadrant order (d = 0: plain column-major);
+// the fast path needs m0 % 128 == 0 so every tile and K slice lies in one leaf.
 template <bool CHECK>
 __global__ __launch_bounds__(NT) void dgemm_kernel(const GemmPtrs *__restrict__ tab, int m,
-                                                   TileMap map) {
+                                                   TileMap map, int m0, int d) {
   __shared__ double As[2][BK * SA];
   __shared__ double Bs[2][BN * SB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -101,12 +106,22 @@ __global__ __launch_bounds__(NT) void dgemm_kernel(const GemmPtrs *__restrict__ 
       for (int c = 0; c < 4; c++) acc[r][c] = v4d{0.0, 0.0, 0.0, 0.0};
 
     v2d ra[4], rb[4];
-    gload_f64<CHECK>(A, B, m, row0, col0, 0, tid, ra, rb);
+    const long long mm0 = (long long)m0 * m0;
+    // first element of the A slice (row0, k0) and B slice (k0, col0)
+    auto a_at = [&](int k0) {
+      return A + quad_leaf(row0 / m0, k0 / m0, d) * mm0 + (row0 % m0) + (long long)(k0 % m0) * m0;
+    };
+    auto b_at = [&](int k0) {
+      return B + quad_leaf(k0 / m0, col0 / m0, d) * mm0 + (k0 % m0) + (long long)(col0 % m0) * m0;
+    };
+    gload_f64<CHECK>(a_at(0), b_at(0), m0, m, row0, col0, 0, tid, ra, rb);
     lstore_f64(As[0], Bs[0], tid, ra, rb);
     __syncthreads();
     for (int kt = 0; kt < nk; kt++) {
       const int buf = kt & 1;
-      if (kt + 1 < nk) gload_f64<CHECK>(A, B, m, row0, col0, (kt + 1) * BK, tid, ra, rb);
+      if (kt + 1 < nk)
+        gload_f64<CHECK>(a_at((kt + 1) * BK), b_at((kt + 1) * BK), m0, m, row0, col0,
+                         (kt + 1) * BK, tid, ra, rb);
 #pragma unroll
       for (int kk = 0; kk < BK / 4; kk++) {
         const int ka = kk * 4 + lk;
@@ -133,7 +148,8 @@ __global__ __launch_bounds__(NT) void dgemm_kernel(const GemmPtrs *__restrict__ 
       for (int c = 0; c < 4; c++) {
         const int row = row0 + wm * 64 + r * 16 + lk;
         const int col = col0 + wn * 64 + c * 16 + l16;
-        double *dst = C + (long long)col * m + row;
+        double *dst = C + quad_leaf(row0 / m0, col0 / m0, d) * mm0 +
+                      (long long)(col % m0) * m0 + (row % m0);
         if (!CHECK) {
 #pragma unroll
           for (int v = 0; v < 4; v++) dst[4 * v] = acc[r][c][v];
@@ -148,18 +164,22 @@ __global__ __launch_bounds__(NT) void dgemm_kernel(const GemmPtrs *__restrict__ 
 
 }  // namespace
 
-hipError_t launch_dgemm_batched(const GemmPtrs *tab, int count, int m, hipStream_t s) {
+hipError_t launch_dgemm_batched(const GemmPtrs *tab, int count, int m, int levels,
+                                hipStream_t s) {
   if (count <= 0 || m <= 0) return hipSuccess;
   bool ok;
   TileMap map = make_tilemap(m, BM, count, ok);
   if (!ok) return hipErrorInvalidValue;
-  const bool fast = (m % BM == 0);
+  const int m0 = m >> levels;
+  if (levels < 0 || (m0 << levels) != m) return hipErrorInvalidValue;
+  const bool fast = (m0 % BM == 0);
+  if (!fast && levels > 0) return hipErrorInvalidValue;  // quadtree operands need whole tiles
   // persistent grid: 2 blocks per CU, a multiple of the XCD count
   const int G = persistent_grid(2, device_cu_count(), map.total);
   if (fast)
-    hipLaunchKernelGGL((dgemm_kernel<false>), dim3(G), dim3(NT), 0, s, tab, m, map);
+    hipLaunchKernelGGL((dgemm_kernel<false>), dim3(G), dim3(NT), 0, s, tab, m, map, m0, levels);
   else
-    hipLaunchKernelGGL((dgemm_kernel<true>), dim3(G), dim3(NT), 0, s, tab, m, map);
+    hipLaunchKernelGGL((dgemm_kernel<true>), dim3(G), dim3(NT), 0, s, tab, m, map, m0, levels);
   return hipGetLastError();
 }
 

@@ -58,9 +58,11 @@ struct GemmPtrs {
   double *C;
 };
 
-// C_p = A_p * B_p for every table entry, m x m column-major, beta = 0.
-// One persistent launch over all tiles of all problems.
-hipError_t launch_dgemm_batched(const GemmPtrs *d_tab, int count, int m,
+// C_p = A_p * B_p for every table entry, beta = 0.  Operands are m x m
+// quadtrees of `levels` quadrant levels over column-major leaves of size
+// m >> levels (levels = 0: plain column-major; levels > 0 needs leaves that
+// are multiples of 128).  One persistent launch over all tiles of all problems.
+hipError_t launch_dgemm_batched(const GemmPtrs *d_tab, int count, int m, int levels,
                                 hipStream_t s);
 
 int device_cu_count();

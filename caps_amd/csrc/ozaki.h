@@ -82,8 +82,10 @@ hipError_t launch_ozaki_step(int step, const OzLeaf *d_tab, int count, int m, in
                              const OzakiConsts *d_consts, hipStream_t s);
 
 // C_l = A_l B_l (mod m_l) for every leaf of the table and every modulus, one
-// persistent launch.
-hipError_t launch_residue_gemm(const RGemm *d_tab, int count, int m, int nmod,
+// persistent launch.  Operands are m x m quadtrees of `levels` quadrant levels
+// over leaves of size m >> levels (A leaves row-major, B and C column-major);
+// levels > 0 needs leaves that are multiples of 128.
+hipError_t launch_residue_gemm(const RGemm *d_tab, int count, int m, int levels, int nmod,
                                const OzakiConsts *d_consts, hipStream_t s);
 
 }  // namespace caps

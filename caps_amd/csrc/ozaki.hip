@@ -187,19 +187,21 @@ constexpr int IBM = 128, IBN = 128, IBK = 64;
 constexpr int ISTR = IBK + 16;  // LDS row stride (bytes): b128 reads of 16 lanes hit 64 banks
 
 // global -> registers: 128 rows of A (k contiguous) and 128 columns of B
-// (k contiguous), 64 k each, 16 bytes per access
+// (k contiguous), 64 k each, 16 bytes per access.  Ab / Bb point at the
+// slice's first element, ld is the row (A) / column (B) stride.  CHECK:
+// bounds against m with the slice origin; only for plain operands.
 template <bool CHECK>
-__device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ap,
-                                         const int8_t *__restrict__ Bp, int m, int row0,
-                                         int col0, int k0, int tid, v4u (&ga)[2],
+__device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ab,
+                                         const int8_t *__restrict__ Bb, long long ld, int m,
+                                         int row0, int col0, int k0, int tid, v4u (&ga)[2],
                                          v4u (&gb)[2]) {
 #pragma unroll
   for (int it = 0; it < 2; it++) {
     const int idx = tid + T256 * it;
     const int r = idx >> 2, kq = (idx & 3) * 16;
     if (!CHECK) {
-      ga[it] = *reinterpret_cast<const v4u *>(Ap + (long long)(row0 + r) * m + k0 + kq);
-      gb[it] = *reinterpret_cast<const v4u *>(Bp + (long long)(col0 + r) * m + k0 + kq);
+      ga[it] = *reinterpret_cast<const v4u *>(Ab + r * ld + kq);
+      gb[it] = *reinterpret_cast<const v4u *>(Bb + r * ld + kq);
     } else {
       unsigned wa[4], wb[4];
 #pragma unroll
@@ -209,9 +211,9 @@ __device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ap,
         for (int b = 0; b < 4; b++) {
           const int k = k0 + kq + 4 * q + b;
           if (k < m && row0 + r < m)
-            va |= (unsigned)(uint8_t)Ap[(long long)(row0 + r) * m + k] << (8 * b);
+            va |= (unsigned)(uint8_t)Ab[r * ld + kq + 4 * q + b] << (8 * b);
           if (k < m && col0 + r < m)
-            vb |= (unsigned)(uint8_t)Bp[(long long)(col0 + r) * m + k] << (8 * b);
+            vb |= (unsigned)(uint8_t)Bb[r * ld + kq + 4 * q + b] << (8 * b);
         }
         wa[q] = va;
         wb[q] = vb;
@@ -234,11 +236,14 @@ __device__ __forceinline__ void lstore_i8(int8_t *As, int8_t *Bs, int tid, const
 }
 
 // problem p = leaf * nmod + l: C_l = A_l * B_l (mod m_l)
+// Operands are m x m quadtrees of d levels over leaves of size m0 = m >> d in
+// CAPS quadrant order: A leaves row-major, B and C leaves column-major (d = 0:
+// plain matrices).  The fast path needs m0 % 128 == 0.
 template <bool CHECK>
 __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict__ tab, int m,
                                                         int nmod,
                                                         const OzakiConsts *__restrict__ kc,
-                                                        TileMap map) {
+                                                        TileMap map, int m0, int d) {
   __shared__ __attribute__((aligned(16))) int8_t As[2][IBM * ISTR];
   __shared__ __attribute__((aligned(16))) int8_t Bs[2][IBN * ISTR];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -267,12 +272,21 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
       for (int c = 0; c < 4; c++) acc[r][c] = v4i{0, 0, 0, 0};
 
     v4u ga[2], gb[2];
-    gload_i8<CHECK>(Ap, Bp, m, row0, col0, 0, tid, ga, gb);
+    const long long mm0 = (long long)m0 * m0;
+    auto a_at = [&](int k0) {  // A slice origin (row0, k0), row-major leaf
+      return Ap + quad_leaf(row0 / m0, k0 / m0, d) * mm0 + (long long)(row0 % m0) * m0 + (k0 % m0);
+    };
+    auto b_at = [&](int k0) {  // B slice origin (k0, col0), column-major leaf
+      return Bp + quad_leaf(k0 / m0, col0 / m0, d) * mm0 + (k0 % m0) + (long long)(col0 % m0) * m0;
+    };
+    gload_i8<CHECK>(a_at(0), b_at(0), m0, m, row0, col0, 0, tid, ga, gb);
     lstore_i8(As[0], Bs[0], tid, ga, gb);
     __syncthreads();
     for (int kt = 0; kt < nk; kt++) {
       const int buf = kt & 1;
-      if (kt + 1 < nk) gload_i8<CHECK>(Ap, Bp, m, row0, col0, (kt + 1) * IBK, tid, ga, gb);
+      if (kt + 1 < nk)
+        gload_i8<CHECK>(a_at((kt + 1) * IBK), b_at((kt + 1) * IBK), m0, m, row0, col0,
+                        (kt + 1) * IBK, tid, ga, gb);
       v4i a[4], b[4];
       // lane group lk supplies k = 16*lk .. 16*lk+15 for both operands
 #pragma unroll
@@ -297,7 +311,8 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
       for (int c = 0; c < 4; c++) {
         const int row = row0 + wm * 64 + r * 16 + 4 * lk;
         const int col = col0 + wn * 64 + c * 16 + l16;
-        int8_t *dst = Cp + (long long)col * m + row;
+        int8_t *dst = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0 +
+                      (long long)(col % m0) * m0 + (row % m0);
         if (!CHECK) {
           unsigned v = 0;
 #pragma unroll
@@ -372,22 +387,25 @@ hipError_t launch_recon(const OzLeaf *tab, int count, int m, int nmod, const Oza
 
 }  // namespace
 
-hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int nmod, const OzakiConsts *kc,
-                               hipStream_t s) {
+hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, int nmod,
+                               const OzakiConsts *kc, hipStream_t s) {
   if (count <= 0 || m <= 0) return hipSuccess;
   if (nmod < 2 || nmod > OZ_MAX_MODULI) return hipErrorInvalidValue;
   bool ok;
   TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);
   if (!ok) return hipErrorInvalidValue;
+  const int m0 = m >> levels;
+  if (levels < 0 || (m0 << levels) != m) return hipErrorInvalidValue;
   const int G = persistent_grid(2, device_cu_count(), map.total);
-  // the fast path reads 16-byte rows: m % 128 == 0 and 16-byte aligned planes
-  bool fast = (m % IBM == 0);
+  // the fast path reads 16-byte rows: leaves of a multiple of 128
+  const bool fast = (m0 % IBM == 0);
+  if (!fast && levels > 0) return hipErrorInvalidValue;
   if (fast)
     hipLaunchKernelGGL((oz_igemm_kernel<false>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
-                       map);
+                       map, m0, levels);
   else
     hipLaunchKernelGGL((oz_igemm_kernel<true>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
-                       map);
+                       map, m0, levels);
   return hipGetLastError();
 }
 

@@ -18,7 +18,7 @@ std::string choose_pattern(const caps_desc &d, int mode, int moduli, long long b
   for (int l = 0; l < d.nrec; l++) {
     std::string trial = pat;
     trial[(size_t)l] = 'b';
-    if (plan_workspace_bytes(d, trial, mode, moduli) <= budget) pat = trial;
+    if (plan_workspace_bytes(d, trial, mode, moduli, 0) <= budget) pat = trial;
   }
   return pat;
 }

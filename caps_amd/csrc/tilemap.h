@@ -45,6 +45,14 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
   return (b % NXCD) * (G / NXCD) + b / NXCD;
 }
 
+// Index, in CAPS quadrant order (11, 21, 12, 22 at each level), of leaf
+// (bi, bj) of a 2^d x 2^d quadtree of leaves.
+__device__ __forceinline__ long long quad_leaf(int bi, int bj, int d) {
+  long long L = 0;
+  for (int l = d - 1; l >= 0; l--) L = 4 * L + (((bi >> l) & 1) + 2 * ((bj >> l) & 1));
+  return L;
+}
+
 inline int persistent_grid(int blocks_per_cu, int cus, int total) {
   int G = cus * blocks_per_cu;
   if (G > total) G = ((total + NXCD - 1) / NXCD) * NXCD;

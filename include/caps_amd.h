@@ -75,7 +75,13 @@ typedef struct caps_opts {
   int mode;      /* caps_mode                                              */
   int moduli;    /* split modes: number of moduli (0 = default 14, 2..16)  */
   int flags;     /* CAPS_FLAG_* below                                      */
-  int reserved;
+  int cutoff;    /* recursion cutoff: the last `cutoff` of the layout's nrec
+                    levels are not Strassen steps; their sub-blocks are
+                    multiplied classically by the leaf GEMM reading the
+                    quadtree layout directly (0 = recurse to the leaves, as
+                    the reference does; > 0 needs leaves that are multiples
+                    of 128, bs = 1 or bs = leaf size, and mode FP64 or
+                    CRT_INT8)                                              */
 } caps_opts;
 
 /* Re-use a captured hipGraph for repeated calls with the same pointers,

@@ -265,3 +265,37 @@ def test_errors_are_reported_not_fatal():
     out = np.zeros(64)
     caps_amd.multiply(np.ones(64), np.ones(64), out, d, "dd")
     assert np.all(out == 8.0)
+
+
+# ------------------------------------------------------------------ recursion cutoff
+@pytest.mark.parametrize("n,nrec,cutoff,pattern", [(512, 2, 2, ""), (512, 2, 1, "b"),
+                                                   (512, 2, 1, "d"), (1024, 3, 2, "b")])
+def test_cutoff_equals_shallower_layout(n, nrec, cutoff, pattern):
+    """Stopping the recursion `cutoff` levels early (classical GEMM over the
+    quadtree of leaves) gives the same bits as the layout with nrec - cutoff
+    levels and full recursion: same pre-additions, same leaf summation order."""
+    rng = np.random.default_rng(n + cutoff)
+    A = rng.uniform(-1, 1, (n, n))
+    B = rng.uniform(-1, 1, (n, n))
+    d = caps_amd.MatDescriptor.one_device(n, nrec, 1)
+    dA, dB = _dev(oracle.to_caps(A, nrec, 1)), _dev(oracle.to_caps(B, nrec, 1))
+    dC = torch.empty_like(dA)
+    print(f"[gpu] cutoff n={n} nrec={nrec} cutoff={cutoff} pattern={pattern}", flush=True)
+    caps_amd.multiply_device(dA, dB, dC, d, pattern + "b" * cutoff, cutoff=cutoff)
+    torch.cuda.synchronize()
+    got = oracle.from_caps(dC.cpu().numpy(), n, nrec, 1)
+    r2 = nrec - cutoff
+    ref = oracle.from_caps(run(oracle.to_caps(A, r2, 1), oracle.to_caps(B, r2, 1), n, r2, 1,
+                               pattern), n, r2, 1)
+    assert np.array_equal(got, ref)
+
+
+def test_cutoff_rejected_where_unsupported():
+    d = caps_amd.MatDescriptor.one_device(256, 2, 1)   # leaves of 64: not a multiple of 128
+    x = torch.zeros(256 * 256, dtype=torch.float64, device="cuda")
+    with pytest.raises(caps_amd.CapsError) as e:
+        caps_amd.multiply_device(x, x, x.clone(), d, "bb", cutoff=1)
+    assert e.value.code == 6
+    with pytest.raises(caps_amd.CapsError) as e:
+        caps_amd.multiply_device(x, x, x.clone(), d, "bb", cutoff=3)
+    assert e.value.code == 1

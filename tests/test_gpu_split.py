@@ -119,3 +119,18 @@ def test_crt_mode_pattern_independent():
     ref = run(a, b, n, r, 1, "bb", CRT)
     for pat in ("dd", "bd", "db", "hb"):
         assert np.array_equal(run(a, b, n, r, 1, pat, CRT), ref), pat
+
+
+def test_crt_mode_cutoff_bit_exact_vs_model():
+    n, nrec = 512, 2
+    A, B = realdata(n, 31)
+    want = split.crt_mode(A, B, 14)
+    d = caps_amd.MatDescriptor.one_device(n, nrec, 1)
+    dA = torch.from_numpy(oracle.to_caps(A, nrec, 1)).cuda()
+    dB = torch.from_numpy(oracle.to_caps(B, nrec, 1)).cuda()
+    for cutoff, pat in ((1, "bb"), (2, "bb"), (1, "db")):
+        dC = torch.empty_like(dA)
+        print(f"[gpu] crt cutoff={cutoff} pattern={pat}", flush=True)
+        caps_amd.multiply_device(dA, dB, dC, d, pat, mode=CRT, moduli=14, cutoff=cutoff)
+        torch.cuda.synchronize()
+        assert np.array_equal(oracle.from_caps(dC.cpu().numpy(), n, nrec, 1), want), cutoff
