@@ -37,6 +37,7 @@ MODE_AUTO = 99
 FLAG_NO_GRAPH = 1
 FLAG_REFERENCE_AUTO = 2
 FLAG_TIMERS = 4
+FLAG_NO_FUSION = 8
 DEFAULT_MODULI = 14
 # phase timers (counters.h:13-18 order, then the split mode's phases)
 TIMER_NAMES = ("comm", "add", "mul", "rearrange", "comm_summa", "split", "recon", "unused")

@@ -174,8 +174,9 @@ int check_cutoff(const caps_desc &d, int mode, int cutoff) {
 }
 
 int get_plan_locked(State &s, const double *A, const double *B, double *C, const caps_desc &d,
-                    const std::string &pat, int mode, int moduli, int cutoff, Plan *&out) {
-  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff};
+                    const std::string &pat, int mode, int moduli, int cutoff, bool fuse,
+                    Plan *&out) {
+  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff, fuse};
   for (auto &p : s.plans)
     if (p->key == key) {
       out = p.get();
@@ -185,7 +186,7 @@ int get_plan_locked(State &s, const double *A, const double *B, double *C, const
   evict_locked(s, MAX_PLANS - 1);
   auto plan = std::make_unique<Plan>();
   plan->key = key;
-  build_plan(*plan, A, B, C, d, pat, mode, moduli, cutoff, nullptr);
+  build_plan(*plan, A, B, C, d, pat, mode, moduli, cutoff, nullptr, fuse);
   const size_t need = plan->work_bytes;
   if (need) {
     size_t freeb = 0, totalb = 0;
@@ -201,7 +202,7 @@ int get_plan_locked(State &s, const double *A, const double *B, double *C, const
     if (e != hipSuccess)
       return fail(CAPS_ENOMEM, "hipMalloc(%zu) for workspace: %s", need, hipGetErrorString(e));
   }
-  build_plan(*plan, A, B, C, d, pat, mode, moduli, cutoff, plan->d_work);
+  build_plan(*plan, A, B, C, d, pat, mode, moduli, cutoff, plan->d_work, fuse);
   if (!plan->host_tables.empty()) {
     hipError_t e = hipMalloc(&plan->d_tables, plan->host_tables.size());
     if (e != hipSuccess) {
@@ -257,7 +258,8 @@ int multiply_device_locked(State &s, const double *dA, const double *dB, double 
   rc = resolve_pattern(*desc, pattern, opts, pat, mode, moduli, s.available_doubles);
   if (rc) return rc;
   Plan *plan = nullptr;
-  rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, cutoff, plan);
+  const bool fuse = !(opts && (opts->flags & CAPS_FLAG_NO_FUSION));
+  rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, cutoff, fuse, plan);
   if (rc) return rc;
   const bool timed = opts && (opts->flags & CAPS_FLAG_TIMERS);
   const bool use_graph = !timed && !(opts && (opts->flags & CAPS_FLAG_NO_GRAPH));

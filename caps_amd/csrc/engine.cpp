@@ -26,13 +26,15 @@
 #include <cstring>
 
 #include "crt.h"
+#include "fused2.h"
 #include "ozaki.h"
 
 namespace caps {
 
 bool PlanKey::operator==(const PlanKey &o) const {
   return A == o.A && B == o.B && C == o.C && memcmp(&desc, &o.desc, sizeof(desc)) == 0 &&
-         pattern == o.pattern && mode == o.mode && moduli == o.moduli && cutoff == o.cutoff;
+         pattern == o.pattern && mode == o.mode && moduli == o.moduli && cutoff == o.cutoff &&
+         fuse == o.fuse;
 }
 
 namespace {
@@ -48,6 +50,7 @@ struct Builder {
   int moduli = 0;
   int cutoff = 0;  // layout levels below the Strassen recursion
   std::string pattern;
+  bool fuse = true;      // fuse two leading 'b' levels when the sizes allow
   bool residue = false;  // matrices are nmod INT8 planes
   int es = 8;            // bytes per element: 8 (FP64) or 1 (residue)
   int planes = 1;        // 1 or nmod
@@ -337,9 +340,86 @@ struct Builder {
     top = mark;
   }
 
+  // Can two 'b' levels starting at `depth` run as fused passes?
+  bool fusable(size_t batch, int s, int nrec, int depth) const {
+    if (!fuse || batch != 1 || nrec - cutoff < 2) return false;
+    if (!(pattern[depth] == 'b' || pattern[depth] == 'B')) return false;
+    if (!(pattern[depth + 1] == 'b' || pattern[depth + 1] == 'B')) return false;
+    const long long q = (long long)(s / 4) * (s / 4);
+    return residue ? (q % 4 == 0) : (q % 2 == 0);
+  }
+
+  // Two fused 'b' levels for one sub-problem (fused2.h).  In residue mode at
+  // the top, `crt` carries the reconstruction target and the combinations are
+  // fused with it.
+  void bfs2(const Prob &pb, int s, int nrec, int depth, const CrtMats *crt) {
+    const int h = s / 2, sub = s / 4;
+    const long long q = (long long)sub * sub;
+    const size_t mark = top;
+    level_tag(depth, pattern[depth]);
+    level_tag(depth + 1, pattern[depth + 1]);
+    plan.additions += 15LL * h * h + 7LL * 15 * q;
+    // plain sub-blocks passed by pointer: level-1 / level-2 operand -> quadrant
+    const int quadA[7] = {0, 2, -1, -1, -1, -1, 3};  // T1 = A11, T2 = A12, T7 = A22
+    const int quadB[7] = {0, 1, -1, -1, -1, 3, -1};  // S1 = B11, S2 = B21, S6 = B22
+    Mat LA = alloc_mats(40, q), LB = alloc_mats(40, q), P2 = alloc_mats(49, q);
+    BBPre pre{};
+    pre.in[0] = pb.A.p;
+    pre.in[1] = pb.B.p;
+    pre.sin[0] = pb.A.ps;
+    pre.sin[1] = pb.B.ps;
+    std::vector<Prob> next(49);
+    int ka = 0, kb = 0;
+    for (int i = 0; i < 7; i++)
+      for (int j = 0; j < 7; j++) {
+        const int k = 7 * i + j;
+        Mat a, b;
+        if (quadA[i] >= 0 && quadA[j] >= 0) {
+          a = at(pb.A, (4 * quadA[i] + quadA[j]) * q);
+        } else {
+          a = at(LA, (long long)ka++ * q);
+          pre.out[0][k] = a.p;
+          pre.sout[0][k] = a.ps;
+        }
+        if (quadB[i] >= 0 && quadB[j] >= 0) {
+          b = at(pb.B, (4 * quadB[i] + quadB[j]) * q);
+        } else {
+          b = at(LB, (long long)kb++ * q);
+          pre.out[1][k] = b.p;
+          pre.sout[1][k] = b.ps;
+        }
+        next[k] = {a, b, at(P2, k * q)};
+      }
+    if (residue) ensure_consts();
+    push(STEP_BB_PRE, residue, 1, q, 0, add_table(&pre, sizeof(pre)));
+    rec(next, sub, nrec - 2, depth + 2);
+    BBPost post{};
+    for (int k = 0; k < 49; k++) {
+      post.q[k] = next[k].C.p;
+      post.sq[k] = next[k].C.ps;
+    }
+    if (!residue) {
+      post.c = reinterpret_cast<double *>(pb.C.p);
+    } else {
+      post.out_c = crt->C;
+      post.sa = crt->sa;
+      post.sb = crt->sb;
+      post.sub = sub;
+      post.m0 = crt->m;
+      post.nrec2 = crt->nrec - 2;
+      post.bs = crt->bs;
+    }
+    push(STEP_BB_POST, residue, 1, q, 0, add_table(&post, sizeof(post)));
+    top = mark;
+  }
+
   void rec(const std::vector<Prob> &batch, int s, int nrec, int depth) {
     if (nrec == cutoff) {
       leaf(batch, s);
+      return;
+    }
+    if (!residue && fusable(batch.size(), s, nrec, depth)) {
+      bfs2(batch[0], s, nrec, depth, nullptr);
       return;
     }
     char c = pattern[depth];
@@ -381,8 +461,13 @@ struct Builder {
     push(STEP_CRT_SCALE, moduli, 1, n, d.bs, mt);
     push(STEP_CRT_SPLIT, moduli, 1, n, d.bs, mt);
     std::vector<Prob> top_batch{{RA, RB, RC}};
-    rec(top_batch, n, d.nrec, 0);
-    push(STEP_CRT_RECON, moduli, 1, n, d.bs, mt);
+    if (fusable(1, n, d.nrec, 0) && (d.bs == 1 || d.bs == m)) {
+      // two fused 'b' levels; the reconstruction is fused with the combinations
+      bfs2(top_batch[0], n, d.nrec, 0, &M);
+    } else {
+      rec(top_batch, n, d.nrec, 0);
+      push(STEP_CRT_RECON, moduli, 1, n, d.bs, mt);
+    }
   }
 };
 
@@ -411,6 +496,12 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
     case STEP_REW:
       return launch_residue_ew(st.op, reinterpret_cast<const RPtrs *>(tb + st.table), st.count,
                                st.n, plan.oz_moduli, st.vec2 != 0, kc, s);
+    case STEP_BB_PRE:
+      return launch_bb_pre(reinterpret_cast<const BBPre *>(tb + st.table), st.n, st.op != 0,
+                           plan.oz_moduli, kc, s);
+    case STEP_BB_POST:
+      return launch_bb_post(reinterpret_cast<const BBPost *>(tb + st.table), st.n, st.op != 0,
+                            st.op ? ozaki_npieces(plan.oz_moduli) : 0, kc, s);
     case STEP_CRT_SCALE:
     case STEP_CRT_SPLIT:
     case STEP_CRT_RECON: {
@@ -423,10 +514,13 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
   return hipErrorInvalidValue;
 }
 
-int timer_of(int kind) {
-  switch (kind) {
+int timer_of(const Step &st) {
+  if (st.kind == STEP_BB_POST && st.op == 0) return CAPS_TIMER_ADD;
+  switch (st.kind) {
     case STEP_EW:
-    case STEP_REW: return CAPS_TIMER_ADD;
+    case STEP_REW:
+    case STEP_BB_PRE: return CAPS_TIMER_ADD;
+    case STEP_BB_POST: return CAPS_TIMER_RECON;
     case STEP_GEMM_F64:
     case STEP_OZ_GEMM: return CAPS_TIMER_MUL;
     case STEP_REORDER: return CAPS_TIMER_REARRANGE;
@@ -443,7 +537,8 @@ int timer_of(int kind) {
 }  // namespace
 
 int build_plan(Plan &plan, const double *A, const double *B, double *C, const caps_desc &d,
-               const std::string &pattern, int mode, int moduli, int cutoff, void *work_base) {
+               const std::string &pattern, int mode, int moduli, int cutoff, void *work_base,
+               bool fuse) {
   plan.steps.clear();
   plan.host_tables.clear();
   plan.additions = plan.multiplications = plan.leaves = 0;
@@ -455,6 +550,7 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
   b.mode = mode;
   b.moduli = moduli;
   b.cutoff = cutoff;
+  b.fuse = fuse;
   b.pattern = pattern;
   b.residue = (mode == CAPS_MODE_CRT_INT8);
   b.es = b.residue ? 1 : 8;
@@ -474,7 +570,7 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
 size_t plan_workspace_bytes(const caps_desc &d, const std::string &pattern, int mode,
                             int moduli, int cutoff) {
   Plan p;
-  build_plan(p, nullptr, nullptr, nullptr, d, pattern, mode, moduli, cutoff, nullptr);
+  build_plan(p, nullptr, nullptr, nullptr, d, pattern, mode, moduli, cutoff, nullptr, true);
   return p.work_bytes;
 }
 
@@ -504,7 +600,7 @@ hipError_t run_steps_timed(const Plan &plan, hipStream_t s, double *ms, long lon
   for (size_t i = 0; i < ns && e == hipSuccess; i++) {
     float t = 0.f;
     e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
-    const int k = timer_of(plan.steps[i].kind);
+    const int k = timer_of(plan.steps[i]);
     ms[k] += t;
     n[k] += 1;
   }

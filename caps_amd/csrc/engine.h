@@ -36,6 +36,8 @@ enum StepKind : int {
   STEP_CRT_SCALE = 8,  // residue mode: scale exponents of the whole A and B
   STEP_CRT_SPLIT = 9,  // residue mode: A, B -> residue planes
   STEP_CRT_RECON = 10, // residue mode: C residues -> C
+  STEP_BB_PRE = 11,    // two 'b' levels of pre-additions in one pass (op = residue)
+  STEP_BB_POST = 12,   // two 'b' levels of combinations (residue: + reconstruction)
 };
 
 struct Step {
@@ -55,6 +57,7 @@ struct PlanKey {
   caps_desc desc;
   std::string pattern;
   int mode, moduli, cutoff;
+  bool fuse;
   bool operator==(const PlanKey &o) const;
 };
 
@@ -82,7 +85,7 @@ struct Plan {
 // work_base = nullptr computes sizes only.
 int build_plan(Plan &plan, const double *A, const double *B, double *C,
                const caps_desc &d, const std::string &pattern, int mode, int moduli,
-               int cutoff, void *work_base);
+               int cutoff, void *work_base, bool fuse = true);
 
 // Enqueue every step of the plan on `s` (no graph).
 hipError_t run_steps(const Plan &plan, hipStream_t s);

@@ -95,6 +95,9 @@ typedef struct caps_opts {
  * startTimer / stopTimer around each add, dgemm and reorder,
  * counters.cpp:19-31).  Synchronises the stream before returning. */
 #define CAPS_FLAG_TIMERS 4
+/* Run two leading 'b' levels as separate passes instead of the fused
+ * two-level kernels (same results; for testing and timing comparisons). */
+#define CAPS_FLAG_NO_FUSION 8
 
 /* Select the device (HIP ordinal) for the calling thread and check that it
  * is gfx950.  Optional: the first call of any entry point does it for the

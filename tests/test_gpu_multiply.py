@@ -299,3 +299,19 @@ def test_cutoff_rejected_where_unsupported():
     with pytest.raises(caps_amd.CapsError) as e:
         caps_amd.multiply_device(x, x, x.clone(), d, "bb", cutoff=3)
     assert e.value.code == 1
+
+
+# ------------------------------------------------------------------ fused two-level passes
+@pytest.mark.parametrize("n,nrec,pattern", [(512, 2, "bb"), (1024, 3, "bbd"), (256, 4, "bbbb"),
+                                            (32, 5, "bbbbb")])
+def test_fused_levels_bit_identical_to_unfused(n, nrec, pattern):
+    """The fused two-level pre/post passes evaluate the same expression trees
+    as two separate 'b' levels: identical bits on real inputs."""
+    rng = np.random.default_rng(n + nrec)
+    a = rng.uniform(-1, 1, n * n) * np.ldexp(1.0, rng.integers(-8, 8, n * n))
+    b = rng.uniform(-1, 1, n * n)
+    fused = run(a, b, n, nrec, 1, pattern)
+    plain = run(a, b, n, nrec, 1, pattern, flags=caps_amd.FLAG_NO_FUSION)
+    assert np.array_equal(fused, plain)
+    if n <= 64:
+        assert np.array_equal(fused, oracle.multiply(a, b, n, nrec, 1, pattern, 4))

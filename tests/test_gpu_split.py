@@ -134,3 +134,16 @@ def test_crt_mode_cutoff_bit_exact_vs_model():
         caps_amd.multiply_device(dA, dB, dC, d, pat, mode=CRT, moduli=14, cutoff=cutoff)
         torch.cuda.synchronize()
         assert np.array_equal(oracle.from_caps(dC.cpu().numpy(), n, nrec, 1), want), cutoff
+
+
+@pytest.mark.parametrize("n,nrec,pattern", [(512, 2, "bb"), (1024, 3, "bbd"), (256, 3, "bbb")])
+def test_crt_fused_levels_bit_exact(n, nrec, pattern):
+    """Fused two-level residue passes with the reconstruction folded into the
+    combinations: same bits as the unfused residue recursion and the model."""
+    A, B = realdata(n, n + 5)
+    a, b = oracle.to_caps(A, nrec, 1), oracle.to_caps(B, nrec, 1)
+    fused = run(a, b, n, nrec, 1, pattern, CRT, 14)
+    plain = run(a, b, n, nrec, 1, pattern, CRT, 14, flags=caps_amd.FLAG_NO_FUSION)
+    assert np.array_equal(fused, plain)
+    if n <= 512:
+        assert np.array_equal(oracle.from_caps(fused, n, nrec, 1), split.crt_mode(A, B, 14))
