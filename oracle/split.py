@@ -86,25 +86,50 @@ def scale_exponent(v: np.ndarray, L: float) -> int:
     return t
 
 
-def _scaled_ints(M: np.ndarray, exps, axis: int) -> np.ndarray:
-    """trunc(2^exp * M) as Python ints, exps per row (axis 0) or column (1)."""
+def _scaled(M: np.ndarray, exps, axis: int) -> np.ndarray:
+    """trunc(2^exp * M) as FP64 integers (exact), exps per row (axis 0) or
+    column (axis 1)."""
     e = np.asarray(exps, dtype=np.int64)
-    scaled = np.trunc(np.ldexp(M, e[:, None] if axis == 0 else e[None, :]))
-    out = np.empty(M.shape, dtype=object)
-    for idx, x in np.ndenumerate(scaled):
-        out[idx] = int(x)
-    return out
+    return np.trunc(np.ldexp(M, e[:, None] if axis == 0 else e[None, :]))
+
+
+def _pieces(X: np.ndarray):
+    """X = X2 2^38 + X1 2^19 + X0 exactly, |Xp| < 2^19, for FP64 integers
+    |X| < 2^57 (every step is an exact FP64 operation)."""
+    X2 = np.trunc(np.ldexp(X, -38))
+    R = X - np.ldexp(X2, 38)
+    X1 = np.trunc(np.ldexp(R, -19))
+    X0 = R - np.ldexp(X1, 19)
+    return X0, X1, X2
+
+
+def exact_int_matmul(Ai: np.ndarray, Bi: np.ndarray) -> np.ndarray:
+    """Exact product of FP64 integer matrices (|entries| < 2^57, K < 2^15) as
+    a numpy object array of Python ints: nine FP64 GEMMs of 19-bit pieces
+    whose partial sums stay below 2^53 (exact in any order), recombined with
+    exact integers."""
+    assert Ai.shape[1] < 2 ** 15
+    assert np.abs(Ai).max(initial=0) < 2.0 ** 57 and np.abs(Bi).max(initial=0) < 2.0 ** 57
+    pa, pb = _pieces(Ai), _pieces(Bi)
+    to_int = np.vectorize(int, otypes=[object])
+    C = np.zeros((Ai.shape[0], Bi.shape[1]), dtype=object)
+    for p in range(3):
+        for q in range(3):
+            G = pa[p] @ pb[q]
+            C = C + to_int(G) * (1 << (19 * (p + q)))
+    return C
 
 
 def scaled_product(A: np.ndarray, B: np.ndarray, count: int):
     """2^-(sa+sb) RN(A'B') for column-major numpy matrices A (rows scaled) and
-    B (columns scaled); also returns (sa, sb, A', B')."""
+    B (columns scaled); also returns (sa, sb, A', B') with A', B' as FP64
+    integers."""
     L = norm_limit(count)
     sa = [scale_exponent(A[i, :], L) for i in range(A.shape[0])]
     sb = [scale_exponent(B[:, j], L) for j in range(B.shape[1])]
-    Ai = _scaled_ints(A, sa, 0)
-    Bi = _scaled_ints(B, sb, 1)
-    Ci = Ai.dot(Bi)
+    Ai = _scaled(A, sa, 0)
+    Bi = _scaled(B, sb, 1)
+    Ci = exact_int_matmul(Ai, Bi)
     C = np.empty(Ci.shape, dtype=np.float64)
     for (i, j), x in np.ndenumerate(Ci):
         C[i, j] = math.ldexp(float(x), -(sa[i] + sb[j]))  # float(int) rounds to nearest even

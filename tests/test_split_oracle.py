@@ -61,9 +61,7 @@ def test_cauchy_schwarz_bound_holds():
     A = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-10, 11, (n, n)))
     B = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-10, 11, (n, n)))
     _, sa, sb, Ai, Bi = split.scaled_product(A, B, 14)
-    absA = np.vectorize(abs, otypes=[object])(Ai)
-    absB = np.vectorize(abs, otypes=[object])(Bi)
-    bound = absA.dot(absB)
+    bound = split.exact_int_matmul(np.abs(Ai), np.abs(Bi))
     M = split.modulus_product(14)
     assert max(bound.ravel()) < M // 2
     # and the scaling used most of the range (within 2 bits per side)
@@ -96,6 +94,15 @@ def test_numpy_winograd_matches_c_oracle():
     # a swapped product would not be
     bad = split.winograd(A, B, r, lambda a, b: oracle.dgemm_naive(b, a))
     assert not np.array_equal(bad, ref)
+
+
+def test_exact_int_matmul_against_python_ints():
+    rng = np.random.default_rng(8)
+    A = np.trunc(rng.uniform(-1, 1, (9, 13)) * 2.0 ** 56)
+    B = np.trunc(rng.uniform(-1, 1, (13, 7)) * 2.0 ** 56)
+    want = np.array([[sum(int(A[i, k]) * int(B[k, j]) for k in range(13)) for j in range(7)]
+                     for i in range(9)], dtype=object)
+    assert (split.exact_int_matmul(A, B) == want).all()
 
 
 def test_crt_mode_error_level():
