@@ -80,6 +80,22 @@ class Counters(ctypes.Structure):
                 ("timers_ms", ctypes.c_double * 8), ("timed_launches", ctypes.c_longlong * 8)]
 
 
+class Candidate(ctypes.Structure):
+    """caps_candidate: one configuration of the autotuner / planner."""
+    _fields_ = [("mode", ctypes.c_int), ("moduli", ctypes.c_int), ("cutoff", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("pattern", ctypes.c_char * 32),
+                ("ms", ctypes.c_double), ("err", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"mode": self.mode, "moduli": self.moduli, "cutoff": self.cutoff,
+                "pattern": self.pattern.decode(), "ms": self.ms, "err": self.err}
+
+
+MASK_FP64 = 1 << MODE_FP64
+MASK_OZAKI_INT8 = 1 << MODE_OZAKI_INT8
+MASK_CRT_INT8 = 1 << MODE_CRT_INT8
+
+
 # every symbol include/caps_amd.h declares, with (restype, argtypes)
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -100,6 +116,21 @@ SIGNATURES = {
     "caps_reset_counters": (ctypes.c_int, []),
     "caps_set_available_memory": (ctypes.c_int, [ctypes.c_longlong]),
     "caps_release": (ctypes.c_int, []),
+    "caps_reference_product": (ctypes.c_int, [_vp, _vp, _vp, _D, _vp]),
+    "caps_normwise_error": (ctypes.c_int, [_vp, _vp, _D, ctypes.POINTER(ctypes.c_double), _vp]),
+    "caps_measure": (ctypes.c_int, [_vp, _vp, _vp, _D, _vp, ctypes.c_int,
+                                    ctypes.POINTER(Candidate), _vp]),
+    "caps_autotune": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_int, ctypes.POINTER(Candidate), ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Candidate), _vp]),
+    "caps_planner_config": (ctypes.c_int, [ctypes.c_double, ctypes.c_int, ctypes.c_double]),
+    "caps_tune_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "caps_tune_load": (ctypes.c_int, [ctypes.c_char_p]),
+    "caps_enumerate_candidates": (ctypes.c_int, [_D, ctypes.c_int, ctypes.POINTER(Candidate),
+                                                 ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "caps_candidate_features": (ctypes.c_int, [_D, ctypes.POINTER(Candidate), ctypes.c_double,
+                                               _vp]),
+    "caps_planner_use_model": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "caps_mlp_nparams": (ctypes.c_int, []),
     "caps_mlp_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_ulonglong]),
     "caps_mlp_destroy": (ctypes.c_int, [_vp]),
@@ -281,6 +312,72 @@ def print_counters(t: float, desc: MatDescriptor) -> str:
     return text
 
 
+# ------------------------------------------------------------------ autotuner
+def candidates(desc: MatDescriptor, mode_mask: int) -> list:
+    n = ctypes.c_int()
+    _check(load().caps_enumerate_candidates(ctypes.byref(desc), mode_mask, None, 0, ctypes.byref(n)))
+    arr = (Candidate * max(1, n.value))()
+    _check(load().caps_enumerate_candidates(ctypes.byref(desc), mode_mask, arr, n.value,
+                                            ctypes.byref(n)))
+    return [arr[i] for i in range(n.value)]
+
+
+def candidate_features(desc: MatDescriptor, cand: Candidate, spread: float) -> np.ndarray:
+    f = np.empty(8, dtype=np.float32)
+    _check(load().caps_candidate_features(ctypes.byref(desc), ctypes.byref(cand), spread,
+                                          f.ctypes.data))
+    return f
+
+
+def reference_product(A, B, Ref, desc: MatDescriptor, stream=None) -> None:
+    n2 = desc.lda * desc.lda
+    _check(load().caps_reference_product(_dev_ptr(A, n2), _dev_ptr(B, n2), _dev_ptr(Ref, n2),
+                                         ctypes.byref(desc), _stream_ptr(stream)))
+
+
+def normwise_error(C, Ref, desc: MatDescriptor, stream=None) -> float:
+    n2 = desc.lda * desc.lda
+    e = ctypes.c_double()
+    _check(load().caps_normwise_error(_dev_ptr(C, n2), _dev_ptr(Ref, n2), ctypes.byref(desc),
+                                      ctypes.byref(e), _stream_ptr(stream)))
+    return e.value
+
+
+def measure(A, B, C, desc: MatDescriptor, cand: Candidate, Ref=None, reps: int = 3,
+            stream=None) -> Candidate:
+    n2 = desc.lda * desc.lda
+    _check(load().caps_measure(_dev_ptr(A, n2), _dev_ptr(B, n2), _dev_ptr(C, n2),
+                               ctypes.byref(desc), _dev_ptr(Ref, n2) if Ref is not None else None,
+                               reps, ctypes.byref(cand), _stream_ptr(stream)))
+    return cand
+
+
+def autotune(A, B, C, desc: MatDescriptor, mode_mask: int, max_err: float, reps: int = 3,
+             stream=None):
+    """Exhaustive search; returns (best, all measured candidates)."""
+    n2 = desc.lda * desc.lda
+    cap = 256
+    out = (Candidate * cap)()
+    n = ctypes.c_int()
+    best = Candidate()
+    _check(load().caps_autotune(_dev_ptr(A, n2), _dev_ptr(B, n2), _dev_ptr(C, n2),
+                                ctypes.byref(desc), mode_mask, max_err, reps, out, cap,
+                                ctypes.byref(n), ctypes.byref(best), _stream_ptr(stream)))
+    return best, [out[i] for i in range(n.value)]
+
+
+def planner_config(max_err: float, mode_mask: int, exponent_spread: float = 10.0) -> None:
+    _check(load().caps_planner_config(max_err, mode_mask, exponent_spread))
+
+
+def tune_save(path: str) -> None:
+    _check(load().caps_tune_save(path.encode()))
+
+
+def tune_load(path: str) -> None:
+    _check(load().caps_tune_load(path.encode()))
+
+
 # ------------------------------------------------------------------ cost / error model
 class MLP:
     """The planner's 3-layer perceptron (8 -> 64 -> 64 -> 2, ReLU, FP32) whose
@@ -332,6 +429,17 @@ class MLP:
         _check(load().caps_mlp_gradient(self._h, X.ctypes.data, T.ctypes.data, M.ctypes.data,
                                         X.shape[0], g.ctypes.data, ctypes.byref(loss)))
         return g, float(loss.value)
+
+    def use_as_planner(self, norm) -> None:
+        """Install as the planner of MODE_AUTO calls (norm: 20 floats, see
+        caps_planner_use_model)."""
+        norm = np.ascontiguousarray(norm, dtype=np.float32)
+        self._norm = norm
+        _check(load().caps_planner_use_model(self._h, norm.ctypes.data, norm.size))
+
+    @staticmethod
+    def clear_planner() -> None:
+        _check(load().caps_planner_use_model(None, None, 0))
 
     def train(self, X, T, M, epochs=200, batch=256, lr=1e-3, seed=0) -> float:
         X, T, M = self._f32(X, self.N_IN), self._f32(T, self.N_OUT), self._f32(M, self.N_OUT)

@@ -20,6 +20,7 @@
 #include "engine.h"
 #include "ozaki.h"
 #include "planner.h"
+#include "tune.h"
 
 namespace caps {
 
@@ -56,6 +57,10 @@ struct State {
   double timers_ms[CAPS_NUM_TIMERS] = {};
   long long timed[CAPS_NUM_TIMERS] = {};
   long long available_doubles = 1LL << 28;  // memory.cpp:4
+  // CAPS_MODE_AUTO: error target, allowed modes, expected exponent spread
+  double auto_max_err = 1e-14;
+  int auto_mask = CAPS_MASK_FP64 | CAPS_MASK_CRT_INT8;
+  double auto_spread = 10.0;
   // host staging buffers (device side) for the host-pointer entry points
   double *stage[3] = {nullptr, nullptr, nullptr};
   size_t stage_elems = 0;
@@ -242,21 +247,59 @@ int capture_graph(Plan &p) {
   return CAPS_OK;
 }
 
+// CAPS_MODE_AUTO: the tuning table, then the learned model, then a default
+// (residue-domain split when allowed: it meets a 1e-14 bar; else FP64)
+int auto_choose(State &s, const caps_desc &d, const char *pattern, int &mode, int &moduli,
+                int &cutoff, std::string &pat) {
+  TuneEntry e;
+  if (tune_lookup(d, s.auto_mask, s.auto_max_err, e)) {
+    mode = e.c.mode;
+    moduli = e.c.moduli ? e.c.moduli : OZ_DEFAULT_MODULI;
+    cutoff = e.c.cutoff;
+    if (!pattern && (int)strlen(e.c.pattern) >= d.nrec) {
+      pat.assign(e.c.pattern, (size_t)d.nrec);
+      return CAPS_OK;
+    }
+  } else if (mlp_planner_choose(d, s.auto_mask, s.auto_max_err, s.auto_spread, mode, moduli,
+                                cutoff, pat)) {
+    if (!pattern) return CAPS_OK;
+  } else {
+    mode = (s.auto_mask & CAPS_MASK_CRT_INT8) ? CAPS_MODE_CRT_INT8 : CAPS_MODE_FP64;
+    moduli = mode == CAPS_MODE_FP64 ? 0 : OZ_DEFAULT_MODULI;
+    cutoff = 0;
+  }
+  if (mode == CAPS_MODE_FP64) moduli = 0;
+  caps_opts o{mode, moduli, 0, cutoff};
+  return resolve_pattern(d, pattern, &o, pat, mode, moduli, s.available_doubles);
+}
+
 int multiply_device_locked(State &s, const double *dA, const double *dB, double *dC,
                            const caps_desc *desc, const char *pattern, const caps_opts *opts,
                            hipStream_t stream) {
   int rc = check_desc(desc);
   if (rc) return rc;
   if (!dA || !dB || !dC) return fail(CAPS_EINVAL, "NULL matrix pointer");
-  int mode, moduli;
-  rc = resolve_mode(*desc, opts, mode, moduli);
-  if (rc) return rc;
-  const int cutoff = opts ? opts->cutoff : 0;
+  int mode, moduli, cutoff;
+  std::string pat;
+  if (opts && opts->mode == CAPS_MODE_AUTO) {
+    rc = auto_choose(s, *desc, pattern, mode, moduli, cutoff, pat);
+    if (rc) return rc;
+    if (pattern) {  // the caller's pattern with the chosen leaf mode
+      caps_opts o{mode, moduli, 0, cutoff};
+      rc = resolve_pattern(*desc, pattern, &o, pat, mode, moduli, s.available_doubles);
+      if (rc) return rc;
+    }
+  } else {
+    rc = resolve_mode(*desc, opts, mode, moduli);
+    if (rc) return rc;
+    cutoff = opts ? opts->cutoff : 0;
+    rc = resolve_pattern(*desc, pattern, opts, pat, mode, moduli, s.available_doubles);
+    if (rc) return rc;
+  }
   rc = check_cutoff(*desc, mode, cutoff);
   if (rc) return rc;
-  std::string pat;
-  rc = resolve_pattern(*desc, pattern, opts, pat, mode, moduli, s.available_doubles);
-  if (rc) return rc;
+  if (mode == CAPS_MODE_CRT_INT8 && desc->nrec > 7)
+    return fail(CAPS_EUNSUPPORTED, "residue-domain mode supports nrec <= 7 (got %d)", desc->nrec);
   Plan *plan = nullptr;
   const bool fuse = !(opts && (opts->flags & CAPS_FLAG_NO_FUSION));
   rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, cutoff, fuse, plan);
@@ -462,6 +505,107 @@ int caps_set_available_memory(long long doubles) {
   std::lock_guard<std::mutex> g(s.mu);
   s.available_doubles = doubles;
   return CAPS_OK;
+}
+
+int caps_reference_product(const double *dA, const double *dB, double *dRef,
+                           const caps_desc *desc, void *stream) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = ensure_device_locked(s);
+  if (rc) return rc;
+  rc = check_desc(desc);
+  if (rc) return rc;
+  // CRT mode with 16 moduli: exact product of operands truncated ~63 bits
+  // below their row / column norms
+  const int nrec = desc->nrec < 7 ? desc->nrec : 7;
+  caps_desc d = *desc;
+  if (nrec != desc->nrec) return fail(CAPS_EUNSUPPORTED, "reference product needs nrec <= 7");
+  caps_opts o{CAPS_MODE_CRT_INT8, 16, 0, 0};
+  std::string pat((size_t)d.nrec, 'd');
+  return multiply_device_locked(s, dA, dB, dRef, &d, pat.c_str(), &o, (hipStream_t)stream);
+}
+
+int caps_normwise_error(const double *dC, const double *dRef, const caps_desc *desc, double *err,
+                        void *stream) {
+  if (!dC || !dRef || !desc || !err) return fail(CAPS_EINVAL, "caps_normwise_error: NULL");
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = ensure_device_locked(s);
+  if (rc) return rc;
+  HIP_TRY(normwise_error(dC, dRef, (long long)desc->lda * desc->lda, err, (hipStream_t)stream));
+  return CAPS_OK;
+}
+
+int caps_measure(const double *dA, const double *dB, double *dC, const caps_desc *desc,
+                 const double *dRef, int reps, caps_candidate *cand, void *stream) {
+  if (!cand || reps < 1) return fail(CAPS_EINVAL, "caps_measure: bad arguments");
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = ensure_device_locked(s);
+  if (rc) return rc;
+  rc = check_desc(desc);
+  if (rc) return rc;
+  hipStream_t hs = (hipStream_t)stream;
+  caps_opts o{cand->mode, cand->moduli, 0, cand->cutoff};
+  std::string pat;
+  if (cand->pattern[0]) {
+    pat = cand->pattern;
+  } else {
+    int mode = cand->mode, moduli = cand->moduli ? cand->moduli : OZ_DEFAULT_MODULI;
+    rc = resolve_pattern(*desc, nullptr, &o, pat, mode, moduli, s.available_doubles);
+    if (rc) return rc;
+    snprintf(cand->pattern, sizeof(cand->pattern), "%s", pat.c_str());
+  }
+  rc = multiply_device_locked(s, dA, dB, dC, desc, pat.c_str(), &o, hs);  // plan + graph
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(hs));
+  std::vector<hipEvent_t> ev(2 * reps);
+  for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  for (int r = 0; r < reps && !rc; r++) {
+    HIP_TRY(hipEventRecord(ev[2 * r], hs));
+    rc = multiply_device_locked(s, dA, dB, dC, desc, pat.c_str(), &o, hs);
+    HIP_TRY(hipEventRecord(ev[2 * r + 1], hs));
+  }
+  HIP_TRY(hipEventSynchronize(ev[2 * reps - 1]));
+  std::vector<double> t(reps);
+  for (int r = 0; r < reps; r++) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
+    t[r] = ms;
+  }
+  for (auto &e : ev) hipEventDestroy(e);
+  if (rc) return rc;
+  std::sort(t.begin(), t.end());
+  cand->ms = t[reps / 2];
+  cand->err = -1.0;
+  if (dRef) {
+    double err = 0;
+    HIP_TRY(normwise_error(dC, dRef, (long long)desc->lda * desc->lda, &err, hs));
+    cand->err = err;
+  }
+  return CAPS_OK;
+}
+
+int caps_planner_config(double max_err, int mode_mask, double exponent_spread) {
+  if (!(max_err > 0) || (mode_mask & ~(CAPS_MASK_FP64 | CAPS_MASK_OZAKI_INT8 | CAPS_MASK_CRT_INT8)) ||
+      !mode_mask)
+    return fail(CAPS_EINVAL, "caps_planner_config: bad error target or mode mask");
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  s.auto_max_err = max_err;
+  s.auto_mask = mode_mask;
+  s.auto_spread = exponent_spread;
+  return CAPS_OK;
+}
+
+int caps_tune_save(const char *path) {
+  if (!path) return fail(CAPS_EINVAL, "NULL path");
+  return tune_save(path);
+}
+
+int caps_tune_load(const char *path) {
+  if (!path) return fail(CAPS_EINVAL, "NULL path");
+  return tune_load(path);
 }
 
 int caps_release(void) {

@@ -14,4 +14,9 @@ std::string choose_pattern(const caps_desc &d, int mode, int moduli, long long b
 // Leaf arithmetic when the caller asks for CAPS_MODE_AUTO.
 int choose_mode(const caps_desc &d);
 
+// The learned planner (tune_plan.cpp): false when no model is installed or no
+// candidate is predicted to meet max_err.
+bool mlp_planner_choose(const caps_desc &d, int mask, double max_err, double spread, int &mode,
+                        int &moduli, int &cutoff, std::string &pat);
+
 }  // namespace caps

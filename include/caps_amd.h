@@ -175,6 +175,57 @@ int caps_set_available_memory(long long doubles);
 int caps_release(void);
 
 /* ------------------------------------------------------------------------
+ * Autotuner (no counterpart in the reference): times candidate configurations
+ * on the device and measures their normwise error max|C - C*| / max|C*|
+ * against a high-precision product C* (CRT mode with 16 moduli); the best
+ * configuration meeting an error target goes into a tuning table consulted by
+ * calls with CAPS_MODE_AUTO.
+ * ------------------------------------------------------------------------ */
+typedef struct caps_candidate {
+  int mode;          /* caps_mode                                           */
+  int moduli;        /* split modes                                         */
+  int cutoff;        /* caps_opts.cutoff                                    */
+  int reserved;
+  char pattern[32];  /* one char per level, NUL-terminated                  */
+  double ms;         /* median milliseconds per multiply                    */
+  double err;        /* normwise error vs the reference product (-1: none)  */
+} caps_candidate;
+
+/* Mode masks for the autotuner and planner */
+#define CAPS_MASK_FP64 (1 << CAPS_MODE_FP64)
+#define CAPS_MASK_OZAKI_INT8 (1 << CAPS_MODE_OZAKI_INT8)
+#define CAPS_MASK_CRT_INT8 (1 << CAPS_MODE_CRT_INT8)
+
+/* High-precision product used as the error reference (device pointers). */
+int caps_reference_product(const double *dA, const double *dB, double *dRef,
+                           const caps_desc *desc, void *stream);
+/* max|C - Ref| / max|Ref| over the n*n entries (device pointers, synchronises). */
+int caps_normwise_error(const double *dC, const double *dRef, const caps_desc *desc,
+                        double *err, void *stream);
+/* Time one candidate (median of `reps` runs after one warm-up, HIP events on
+ * `stream`) and, if dRef is not NULL, its error; fills cand->ms and ->err. */
+int caps_measure(const double *dA, const double *dB, double *dC, const caps_desc *desc,
+                 const double *dRef, int reps, caps_candidate *cand, void *stream);
+/* Exhaustive search over the candidates allowed by mode_mask (pattern over
+ * the recursed levels, cutoff, moduli 12..16 for split modes) on the given
+ * operands; all measured candidates go to out[0..*n_out) (up to max_out),
+ * the fastest one with err <= max_err to *best and to the tuning table. */
+int caps_autotune(const double *dA, const double *dB, double *dC, const caps_desc *desc,
+                  int mode_mask, double max_err, int reps, caps_candidate *out, int max_out,
+                  int *n_out, caps_candidate *best, void *stream);
+/* The error target and allowed modes that CAPS_MODE_AUTO calls look up. */
+int caps_planner_config(double max_err, int mode_mask, double exponent_spread);
+int caps_tune_save(const char *path);
+int caps_tune_load(const char *path);
+/* The candidates caps_autotune would measure, and the 8 model features of a
+ * candidate: log2 n, exponent spread, #b levels, #d levels, cutoff, moduli,
+ * is-residue-mode, is-leaf-split-mode. */
+int caps_enumerate_candidates(const caps_desc *desc, int mode_mask, caps_candidate *out,
+                              int max_out, int *n_out);
+int caps_candidate_features(const caps_desc *desc, const caps_candidate *c, double spread,
+                            float *features8);
+
+/* ------------------------------------------------------------------------
  * Planner cost / error model (no counterpart in the reference, which picks
  * BFS or DFS from a memory rule, multiply.cpp:43-66): a 3-layer perceptron
  * 8 -> 64 -> 64 -> 2 (ReLU, FP32) whose forward, backward and Adam steps run
@@ -203,6 +254,10 @@ int caps_mlp_gradient(caps_mlp *m, const float *X, const float *T, const float *
 int caps_mlp_train(caps_mlp *m, const float *X, const float *T, const float *M, int B,
                    int epochs, int batch, float lr, unsigned long long seed,
                    float *final_loss);
+/* Install m (NULL: remove) as the planner of CAPS_MODE_AUTO calls that have
+ * no tuning-table entry.  norm: feature means[8], feature scales[8], target
+ * means[2], target scales[2]; targets are log2(ms) and log10(error). */
+int caps_planner_use_model(caps_mlp *m, const float *norm, int nnorm);
 
 #ifdef __cplusplus
 }

@@ -81,3 +81,18 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     with pytest.raises(caps_amd.CapsError) as e:
         caps_amd.load(str(tmp_path / "nope.so"))
     assert "no CPU fallback" in str(e.value)
+
+
+def test_candidate_enumeration(lib):
+    D = caps_amd.MatDescriptor.one_device
+    # n = 8192, nrec = 2 (leaves of 2048): cutoffs 0..2, prefix patterns b^k d^(L-k)
+    fp = caps_amd.candidates(D(8192, 2, 1), caps_amd.MASK_FP64)
+    assert sorted((c.cutoff, c.pattern.decode()) for c in fp) == sorted(
+        [(0, "bb"), (0, "bd"), (0, "dd"), (1, "bb"), (1, "db"), (2, "bb")])
+    crt = caps_amd.candidates(D(8192, 2, 1), caps_amd.MASK_CRT_INT8)
+    assert {c.moduli for c in crt} == {12, 13, 14, 15, 16} and len(crt) == 30
+    # leaves not a multiple of 128: no cutoff
+    small = caps_amd.candidates(D(392, 3, 7), caps_amd.MASK_FP64 | caps_amd.MASK_CRT_INT8)
+    assert all(c.cutoff == 0 for c in small)
+    f = caps_amd.candidate_features(D(8192, 2, 1), crt[0], 10.0)
+    assert list(f[:1]) == [13.0] and f[1] == 10.0 and f[6] == 1.0 and f[7] == 0.0
