@@ -108,6 +108,8 @@ SIGNATURES = {
     "caps_num_entries": (ctypes.c_longlong, [_D]),
     "caps_multiply": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_char_p, _O]),
     "caps_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_char_p, _O, _vp]),
+    "caps_multiply_batch_device": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _O,
+                                                  _vp]),
     "caps_to_layout": (ctypes.c_int, [_vp, _vp, _D]),
     "caps_from_layout": (ctypes.c_int, [_vp, _vp, _D]),
     "caps_to_layout_device": (ctypes.c_int, [_vp, _vp, _D, _vp]),
@@ -236,6 +238,26 @@ def multiply_device(A, B, C, desc: MatDescriptor, pattern: str | None = None,
     _check(load().caps_multiply_device(_dev_ptr(A, n2), _dev_ptr(B, n2), _dev_ptr(C, n2),
                                        ctypes.byref(desc), _pattern(pattern), ctypes.byref(o),
                                        _stream_ptr(stream)))
+
+
+def multiply_batch_device(As, Bs, Cs, descs, patterns=None, mode: int = MODE_FP64,
+                          moduli: int = 0, flags: int = 0, stream=None) -> None:
+    """Enqueue independent multiplies C_i = A_i * B_i (device tensors, CAPS
+    layout, per-item descriptors) as one batched plan replayed as one graph."""
+    k = len(As)
+    assert len(Bs) == k and len(Cs) == k and len(descs) == k
+    P = ctypes.c_void_p * k
+    pa = P(*[_dev_ptr(a, d.lda * d.lda) for a, d in zip(As, descs)])
+    pb = P(*[_dev_ptr(b, d.lda * d.lda) for b, d in zip(Bs, descs)])
+    pc = P(*[_dev_ptr(c, d.lda * d.lda) for c, d in zip(Cs, descs)])
+    D = (MatDescriptor * k)(*descs)
+    pats = None
+    if patterns is not None:
+        enc = [_pattern(p) for p in patterns]
+        pats = (ctypes.c_char_p * k)(*enc)
+    o = _opts(mode, moduli, flags)
+    _check(load().caps_multiply_batch_device(k, pa, pb, pc, D, pats, ctypes.byref(o),
+                                             _stream_ptr(stream)))
 
 
 # ------------------------------------------------------------------ layout

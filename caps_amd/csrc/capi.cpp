@@ -181,7 +181,7 @@ int check_cutoff(const caps_desc &d, int mode, int cutoff) {
 int get_plan_locked(State &s, const double *A, const double *B, double *C, const caps_desc &d,
                     const std::string &pat, int mode, int moduli, int cutoff, bool fuse,
                     Plan *&out) {
-  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff, fuse};
+  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff, fuse, std::string()};
   for (auto &p : s.plans)
     if (p->key == key) {
       out = p.get();
@@ -219,6 +219,45 @@ int get_plan_locked(State &s, const double *A, const double *B, double *C, const
     if (e != hipSuccess) {
       free_plan(*plan);
       return fail(CAPS_EHIP, "uploading pointer tables: %s", hipGetErrorString(e));
+    }
+  }
+  plan->last_use = ++s.clock;
+  out = plan.get();
+  s.plans.push_back(std::move(plan));
+  return CAPS_OK;
+}
+
+// allocate, build and upload a plan whose builder is `build(plan, work)`
+template <typename F>
+int make_plan_locked(State &s, const PlanKey &key, F build, Plan *&out) {
+  evict_locked(s, MAX_PLANS - 1);
+  auto plan = std::make_unique<Plan>();
+  plan->key = key;
+  build(*plan, nullptr);
+  const size_t need = plan->work_bytes;
+  if (need) {
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    if (need > freeb) {
+      evict_locked(s, 0);
+      HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+      if (need > freeb)
+        return fail(CAPS_ENOMEM, "plan needs %.1f MiB of device workspace, %.1f MiB free",
+                    need / 1048576.0, freeb / 1048576.0);
+    }
+    hipError_t e = hipMalloc(&plan->d_work, need);
+    if (e != hipSuccess)
+      return fail(CAPS_ENOMEM, "hipMalloc(%zu) for workspace: %s", need, hipGetErrorString(e));
+  }
+  build(*plan, plan->d_work);
+  if (!plan->host_tables.empty()) {
+    hipError_t e = hipMalloc(&plan->d_tables, plan->host_tables.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(plan->d_tables, plan->host_tables.data(), plan->host_tables.size(),
+                    hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      free_plan(*plan);
+      return fail(CAPS_EHIP, "pointer tables: %s", hipGetErrorString(e));
     }
   }
   plan->last_use = ++s.clock;
@@ -504,6 +543,84 @@ int caps_set_available_memory(long long doubles) {
   State &s = st();
   std::lock_guard<std::mutex> g(s.mu);
   s.available_doubles = doubles;
+  return CAPS_OK;
+}
+
+int caps_multiply_batch_device(int count, const double *const *dA, const double *const *dB,
+                               double *const *dC, const caps_desc *descs,
+                               const char *const *patterns, const caps_opts *opts,
+                               void *stream) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = ensure_device_locked(s);
+  if (rc) return rc;
+  if (count <= 0 || !dA || !dB || !dC || !descs)
+    return fail(CAPS_EINVAL, "caps_multiply_batch_device: bad arguments");
+  if (opts && opts->mode == CAPS_MODE_AUTO)
+    return fail(CAPS_EUNSUPPORTED, "batched calls take an explicit leaf mode");
+  int mode, moduli;
+  rc = resolve_mode(descs[0], opts, mode, moduli);
+  if (rc) return rc;
+  const int cutoff = opts ? opts->cutoff : 0;
+  std::vector<BatchItem> items((size_t)count);
+  std::string key;
+  key.reserve((size_t)count * 64);
+  for (int i = 0; i < count; i++) {
+    rc = check_desc(&descs[i]);
+    if (rc) return rc;
+    if (!dA[i] || !dB[i] || !dC[i]) return fail(CAPS_EINVAL, "NULL matrix pointer in item %d", i);
+    rc = check_cutoff(descs[i], mode, cutoff);
+    if (rc) return rc;
+    if (mode == CAPS_MODE_CRT_INT8 && descs[i].nrec > 7)
+      return fail(CAPS_EUNSUPPORTED, "residue-domain mode supports nrec <= 7");
+    items[i].A = dA[i];
+    items[i].B = dB[i];
+    items[i].C = dC[i];
+    items[i].desc = descs[i];
+    caps_opts o{mode, moduli, 0, cutoff};
+    rc = resolve_pattern(descs[i], patterns ? patterns[i] : nullptr, &o, items[i].pattern, mode,
+                         moduli, s.available_doubles / count);
+    if (rc) return rc;
+    char buf[160];
+    snprintf(buf, sizeof(buf), "%p %p %p %d %d %d %s;", (const void *)dA[i], (const void *)dB[i],
+             (void *)dC[i], descs[i].lda, descs[i].nrec, descs[i].bs, items[i].pattern.c_str());
+    key += buf;
+  }
+  PlanKey pk{nullptr, nullptr, nullptr, descs[0], std::string(), mode, moduli, cutoff, true, key};
+  Plan *plan = nullptr;
+  for (auto &p : s.plans)
+    if (p->key == pk) {
+      plan = p.get();
+      p->last_use = ++s.clock;
+    }
+  if (!plan) {
+    rc = make_plan_locked(
+        s, pk,
+        [&](Plan &pl, void *work) {
+          build_batch_plan(pl, items, mode, moduli, cutoff, work);
+        },
+        plan);
+    if (rc) return rc;
+  }
+  const bool use_graph = !(opts && (opts->flags & (CAPS_FLAG_NO_GRAPH | CAPS_FLAG_TIMERS)));
+  if (opts && (opts->flags & CAPS_FLAG_TIMERS)) {
+    hipError_t e = run_steps_timed(*plan, (hipStream_t)stream, s.timers_ms, s.timed);
+    if (e != hipSuccess) return fail(CAPS_EHIP, "timed launch: %s", hipGetErrorString(e));
+  } else if (use_graph) {
+    if (!plan->graph) {
+      rc = capture_graph(*plan);
+      if (rc) return rc;
+    }
+    HIP_TRY(hipGraphLaunch(plan->graph, (hipStream_t)stream));
+  } else {
+    hipError_t e = run_steps(*plan, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(CAPS_EHIP, "kernel launch: %s", hipGetErrorString(e));
+  }
+  s.additions += plan->additions;
+  s.multiplications += plan->multiplications;
+  s.leaves += plan->leaves;
+  s.peak_mb = plan->work_bytes / 1048576.0;
+  s.exec_pattern = plan->exec_pattern;
   return CAPS_OK;
 }
 

@@ -34,7 +34,7 @@ namespace caps {
 bool PlanKey::operator==(const PlanKey &o) const {
   return A == o.A && B == o.B && C == o.C && memcmp(&desc, &o.desc, sizeof(desc)) == 0 &&
          pattern == o.pattern && mode == o.mode && moduli == o.moduli && cutoff == o.cutoff &&
-         fuse == o.fuse;
+         fuse == o.fuse && batch == o.batch;
 }
 
 namespace {
@@ -430,43 +430,54 @@ struct Builder {
       dfs(batch, s, nrec, depth);
   }
 
-  // residue mode: scale and split A, B once, recurse on residues, reconstruct
-  void crt_top(const double *A, const double *B, double *C, const caps_desc &d) {
+  // residue mode: scale and split every A, B of the group once, recurse on
+  // the residues of the whole group, reconstruct each C
+  void crt_group(const std::vector<const BatchItem *> &items) {
+    const caps_desc &d = items[0]->desc;
     const int n = d.lda, m = n >> d.nrec;
     const long long n2 = (long long)n * n;
     ensure_consts();
-    CrtMats M{};
-    M.A = A;
-    M.B = B;
-    M.C = C;
-    M.n = n;
-    M.nrec = d.nrec;
-    M.bs = d.bs;
-    M.m = m;
-    M.nmod = moduli;
-    M.chunks = crt_row_chunks(m);
-    const int nblk = 1 << d.nrec;
-    M.sa = reinterpret_cast<int *>(base + alloc(sizeof(int) * n));
-    M.sb = reinterpret_cast<int *>(base + alloc(sizeof(int) * n));
-    M.pa = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk * M.chunks));
-    M.pb = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk));
-    Mat RA = alloc_mats(1, n2), RB = alloc_mats(1, n2), RC = alloc_mats(1, n2);
-    M.ra = reinterpret_cast<int8_t *>(RA.p);
-    M.rb = reinterpret_cast<int8_t *>(RB.p);
-    M.rc = reinterpret_cast<int8_t *>(RC.p);
-    const size_t mt = add_table(&M, sizeof(M));
-    plan.crt_mats = mt;
-    plan.crt_host.assign(reinterpret_cast<unsigned char *>(&M),
-                         reinterpret_cast<unsigned char *>(&M) + sizeof(M));
-    push(STEP_CRT_SCALE, moduli, 1, n, d.bs, mt);
-    push(STEP_CRT_SPLIT, moduli, 1, n, d.bs, mt);
-    std::vector<Prob> top_batch{{RA, RB, RC}};
-    if (fusable(1, n, d.nrec, 0) && (d.bs == 1 || d.bs == m)) {
+    std::vector<Prob> top_batch;
+    std::vector<size_t> mats;
+    for (const BatchItem *it : items) {
+      CrtMats M{};
+      M.A = it->A;
+      M.B = it->B;
+      M.C = it->C;
+      M.n = n;
+      M.nrec = d.nrec;
+      M.bs = d.bs;
+      M.m = m;
+      M.nmod = moduli;
+      M.chunks = crt_row_chunks(m);
+      const int nblk = 1 << d.nrec;
+      M.sa = reinterpret_cast<int *>(base + alloc(sizeof(int) * n));
+      M.sb = reinterpret_cast<int *>(base + alloc(sizeof(int) * n));
+      M.pa = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk * M.chunks));
+      M.pb = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk));
+      Mat RA = alloc_mats(1, n2), RB = alloc_mats(1, n2), RC = alloc_mats(1, n2);
+      M.ra = reinterpret_cast<int8_t *>(RA.p);
+      M.rb = reinterpret_cast<int8_t *>(RB.p);
+      M.rc = reinterpret_cast<int8_t *>(RC.p);
+      const size_t mt = add_table(&M, sizeof(M));
+      plan.crt_host.push_back(M);
+      const size_t hi = plan.crt_host.size() - 1;
+      push(STEP_CRT_SCALE, moduli, 1, n, d.bs, mt);
+      plan.steps.back().aux = hi;
+      push(STEP_CRT_SPLIT, moduli, 1, n, d.bs, mt);
+      plan.steps.back().aux = hi;
+      top_batch.push_back({RA, RB, RC});
+      mats.push_back(mt);
+    }
+    if (items.size() == 1 && fusable(1, n, d.nrec, 0) && (d.bs == 1 || d.bs == m)) {
       // two fused 'b' levels; the reconstruction is fused with the combinations
-      bfs2(top_batch[0], n, d.nrec, 0, &M);
-    } else {
-      rec(top_batch, n, d.nrec, 0);
-      push(STEP_CRT_RECON, moduli, 1, n, d.bs, mt);
+      bfs2(top_batch[0], n, d.nrec, 0, &plan.crt_host.back());
+      return;
+    }
+    rec(top_batch, n, d.nrec, 0);
+    for (size_t i = 0; i < items.size(); i++) {
+      push(STEP_CRT_RECON, moduli, 1, n, d.bs, mats[i]);
+      plan.steps.back().aux = plan.crt_host.size() - items.size() + i;
     }
   }
 };
@@ -504,12 +515,10 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
                             st.op ? ozaki_npieces(plan.oz_moduli) : 0, kc, s);
     case STEP_CRT_SCALE:
     case STEP_CRT_SPLIT:
-    case STEP_CRT_RECON: {
-      CrtMats h;
-      memcpy(&h, plan.crt_host.data(), sizeof(h));
+    case STEP_CRT_RECON:
       return launch_crt_step(st.kind - STEP_CRT_SCALE,
-                             reinterpret_cast<const CrtMats *>(tb + st.table), h, kc, s);
-    }
+                             reinterpret_cast<const CrtMats *>(tb + st.table),
+                             plan.crt_host[st.aux], kc, s);
   }
   return hipErrorInvalidValue;
 }
@@ -556,7 +565,8 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
   b.es = b.residue ? 1 : 8;
   b.planes = b.residue ? moduli : 1;
   if (b.residue) {
-    b.crt_top(A, B, C, d);
+    BatchItem it{A, B, C, d, pattern};
+    b.crt_group({&it});
   } else {
     std::vector<Prob> top{{{reinterpret_cast<char *>(const_cast<double *>(A)), 0},
                            {reinterpret_cast<char *>(const_cast<double *>(B)), 0},
@@ -564,6 +574,55 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
     b.rec(top, d.lda, d.nrec, 0);
   }
   plan.work_bytes = b.peak;
+  return CAPS_OK;
+}
+
+int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, int moduli,
+                     int cutoff, void *work_base) {
+  plan.steps.clear();
+  plan.host_tables.clear();
+  plan.additions = plan.multiplications = plan.leaves = 0;
+  plan.exec_pattern.clear();
+  plan.oz_moduli = 0;
+  plan.crt_host.clear();
+  // group by (descriptor, pattern), in first-appearance order
+  std::vector<std::vector<const BatchItem *>> groups;
+  for (const BatchItem &it : items) {
+    bool placed = false;
+    for (auto &g : groups)
+      if (memcmp(&g[0]->desc, &it.desc, sizeof(caps_desc)) == 0 && g[0]->pattern == it.pattern) {
+        g.push_back(&it);
+        placed = true;
+        break;
+      }
+    if (!placed) groups.push_back({&it});
+  }
+  size_t peak = 0;
+  for (auto &g : groups) {
+    Builder b{plan, reinterpret_cast<char *>(work_base)};
+    b.bs = g[0]->desc.bs;
+    b.mode = mode;
+    b.moduli = moduli;
+    b.cutoff = cutoff;
+    b.pattern = g[0]->pattern;
+    b.residue = (mode == CAPS_MODE_CRT_INT8);
+    b.es = b.residue ? 1 : 8;
+    b.planes = b.residue ? moduli : 1;
+    b.fuse = true;
+    if (b.residue) {
+      b.crt_group(g);
+    } else {
+      std::vector<Prob> batch;
+      for (const BatchItem *it : g)
+        batch.push_back({{reinterpret_cast<char *>(const_cast<double *>(it->A)), 0},
+                         {reinterpret_cast<char *>(const_cast<double *>(it->B)), 0},
+                         {reinterpret_cast<char *>(it->C), 0}});
+      b.rec(batch, g[0]->desc.lda, g[0]->desc.nrec, 0);
+    }
+    peak = std::max(peak, b.peak);
+  }
+  plan.work_bytes = peak;
+  plan.exec_pattern = "BATCH,";
   return CAPS_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/caps_amd.h"
+#include "crt.h"
 #include "kernels.h"
 
 namespace caps {
@@ -58,6 +59,7 @@ struct PlanKey {
   std::string pattern;
   int mode, moduli, cutoff;
   bool fuse;
+  std::string batch;  // batched plans: every item's pointers, descriptor and pattern
   bool operator==(const PlanKey &o) const;
 };
 
@@ -76,9 +78,18 @@ struct Plan {
   // split-mode constants (moduli, CRT weights), device copy in d_tables
   size_t oz_const = 0;
   int oz_moduli = 0;
-  // residue mode: host copy of the top-level residue description
-  size_t crt_mats = 0;
-  std::vector<unsigned char> crt_host;
+  // residue mode: host copies of the residue descriptions (Step.aux indexes)
+  std::vector<CrtMats> crt_host;
+  // batched plans: the key of every item
+  std::string batch_key;
+};
+
+// one independent multiply of a batched call
+struct BatchItem {
+  const double *A, *B;
+  double *C;
+  caps_desc desc;
+  std::string pattern;  // resolved, one char per level
 };
 
 // Build a plan: pattern must already be resolved to one char per level.
@@ -86,6 +97,12 @@ struct Plan {
 int build_plan(Plan &plan, const double *A, const double *B, double *C,
                const caps_desc &d, const std::string &pattern, int mode, int moduli,
                int cutoff, void *work_base, bool fuse = true);
+
+// Build one plan for independent multiplies of possibly different shapes:
+// items with the same (descriptor, pattern) share batched steps; groups run
+// one after another and reuse the same workspace.
+int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, int moduli,
+                     int cutoff, void *work_base);
 
 // Enqueue every step of the plan on `s` (no graph).
 hipError_t run_steps(const Plan &plan, hipStream_t s);

@@ -129,6 +129,16 @@ int caps_multiply_device(const double *dA, const double *dB, double *dC,
                          const caps_desc *desc, const char *pattern,
                          const caps_opts *opts, void *stream);
 
+/* `count` independent multiplies C_i = A_i * B_i of possibly different
+ * sizes and depths (device pointers, CAPS layout each), one leaf mode for
+ * all (opts; CAPS_MODE_AUTO is not accepted), patterns[i] or NULL per item.
+ * Items with the same descriptor and pattern share batched launches; all
+ * launches of the call go into one captured hipGraph replayed as one launch. */
+int caps_multiply_batch_device(int count, const double *const *dA, const double *const *dB,
+                               double *const *dC, const caps_desc *descs,
+                               const char *const *patterns, const caps_opts *opts,
+                               void *stream);
+
 /* Column-major n x n <-> CAPS layout, one process
  * (distributeFrom1Proc / collectTo1Proc, communication.cpp:612-708). */
 int caps_to_layout(const double *colmajor, double *caps, const caps_desc *desc);

@@ -1,0 +1,92 @@
+"""GPU parity of the uneven batch entry point (caps_multiply_batch_device):
+a set of independent problems of different sizes, depths and patterns runs as
+one captured graph, and each result equals the same problem run alone (FP64:
+the oracle's bits; residue-domain mode: the exact-integer model's bits)."""
+import numpy as np
+import pytest
+
+import caps_amd
+import oracle
+from oracle import split
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _setup():
+    oracle.build()
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    caps_amd.load()
+    yield
+    caps_amd.release()
+
+
+# (n, nrec, bs, pattern): repeated shapes share batched launches
+ITEMS = [(64, 2, 1, "bb"), (128, 2, 1, "bd"), (64, 2, 1, "bb"), (56, 3, 1, "bbb"),
+         (192, 1, 1, "b"), (128, 2, 8, "dd"), (64, 2, 1, "bb")]
+
+
+def _problems(seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n, r, bs, pat in ITEMS:
+        A = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-8, 9, (n, n)))
+        B = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-8, 9, (n, n)))
+        out.append((A, B, n, r, bs, pat))
+    return out
+
+
+def _run_batch(probs, mode, moduli=0, flags=0):
+    print(f"[gpu] batch of {len(probs)} mode={mode} flags={flags}", flush=True)
+    As, Bs, Cs, descs, pats = [], [], [], [], []
+    for A, B, n, r, bs, pat in probs:
+        As.append(torch.from_numpy(oracle.to_caps(A, r, bs)).cuda())
+        Bs.append(torch.from_numpy(oracle.to_caps(B, r, bs)).cuda())
+        Cs.append(torch.full_like(As[-1], float("nan")))
+        descs.append(caps_amd.MatDescriptor.one_device(n, r, bs))
+        pats.append(pat)
+    caps_amd.multiply_batch_device(As, Bs, Cs, descs, pats, mode, moduli, flags)
+    torch.cuda.synchronize()
+    return [oracle.from_caps(c.cpu().numpy(), p[2], p[3], p[4]) for c, p in zip(Cs, probs)]
+
+
+@pytest.mark.parametrize("flags", [0, caps_amd.FLAG_NO_GRAPH])
+def test_batch_fp64_matches_oracle(flags):
+    probs = _problems(1)
+    got = _run_batch(probs, caps_amd.MODE_FP64, 0, flags)
+    for (A, B, n, r, bs, pat), g in zip(probs, got):
+        want = oracle.from_caps(oracle.multiply(oracle.to_caps(A, r, bs), oracle.to_caps(B, r, bs),
+                                                n, r, bs, pat, 2), n, r, bs)
+        assert np.array_equal(g, want), (n, r, bs, pat)
+
+
+def test_batch_crt_matches_model():
+    probs = _problems(2)
+    got = _run_batch(probs, caps_amd.MODE_CRT_INT8, 14)
+    for (A, B, n, r, bs, pat), g in zip(probs, got):
+        assert np.array_equal(g, split.crt_mode(A, B, 14)), (n, r, bs, pat)
+
+
+def test_batch_replay_and_single_item():
+    """Replaying the cached batch graph on new data, and a batch of one."""
+    probs = _problems(3)
+    first = _run_batch(probs, caps_amd.MODE_FP64)
+    again = _run_batch(probs, caps_amd.MODE_FP64)
+    for a, b in zip(first, again):
+        assert np.array_equal(a, b)
+    A, B, n, r, bs, pat = probs[1]
+    one = _run_batch([probs[1]], caps_amd.MODE_FP64)[0]
+    assert np.array_equal(one, first[1])
+
+
+def test_batch_rejects_auto_and_bad_items():
+    d = caps_amd.MatDescriptor.one_device(64, 2, 1)
+    x = torch.zeros(64 * 64, dtype=torch.float64, device="cuda")
+    with pytest.raises(caps_amd.CapsError):
+        caps_amd.multiply_batch_device([x], [x], [x], [d], None, caps_amd.MODE_AUTO)
+    bad = caps_amd.MatDescriptor.one_device(64, 2, 3)   # 16 % 3 != 0
+    with pytest.raises(caps_amd.CapsError):
+        caps_amd.multiply_batch_device([x], [x], [x], [bad], None, caps_amd.MODE_FP64)
