@@ -47,8 +47,11 @@ __device__ __forceinline__ int scale_exponent(int e, double s, double L) {
 
 // symmetric residue of the FP64 integer x (|x| < 2^60) modulo mi
 __device__ __forceinline__ int sym_residue(double x, int mi, double inv) {
+  // |x| <= 2^63: q is within 8 of x/m, so r = x - q m is a small integer
+  // (|r| < 9m, exact in one fma); a second exact step brings it to |r| <= m
   const double q = rint(x * inv);
-  const double r = fma(-q, (double)mi, x);  // exact: |r| < m
+  double r = fma(-q, (double)mi, x);
+  r = fma(-rint(r * inv), (double)mi, r);
   int ri = (int)r;
   const int hi = (mi - 1) >> 1, lo = -(mi >> 1);
   ri = ri > hi ? ri - mi : ri;

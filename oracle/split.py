@@ -93,28 +93,33 @@ def _scaled(M: np.ndarray, exps, axis: int) -> np.ndarray:
     return np.trunc(np.ldexp(M, e[:, None] if axis == 0 else e[None, :]))
 
 
-def _pieces(X: np.ndarray):
-    """X = X2 2^38 + X1 2^19 + X0 exactly, |Xp| < 2^19, for FP64 integers
-    |X| < 2^57 (every step is an exact FP64 operation)."""
-    X2 = np.trunc(np.ldexp(X, -38))
-    R = X - np.ldexp(X2, 38)
-    X1 = np.trunc(np.ldexp(R, -19))
-    X0 = R - np.ldexp(X1, 19)
-    return X0, X1, X2
+def _pieces(X: np.ndarray, count: int):
+    """X = sum_p X_p 2^(19 p) exactly, |X_p| < 2^19, for FP64 integers
+    |X| < 2^(19 count) (every step is an exact FP64 operation)."""
+    out = []
+    R = X
+    for p in range(count - 1, 0, -1):
+        Xp = np.trunc(np.ldexp(R, -19 * p))
+        R = R - np.ldexp(Xp, 19 * p)
+        out.append(Xp)
+    out.append(R)
+    return out[::-1]
 
 
 def exact_int_matmul(Ai: np.ndarray, Bi: np.ndarray) -> np.ndarray:
-    """Exact product of FP64 integer matrices (|entries| < 2^57, K < 2^15) as
-    a numpy object array of Python ints: nine FP64 GEMMs of 19-bit pieces
-    whose partial sums stay below 2^53 (exact in any order), recombined with
-    exact integers."""
+    """Exact product of FP64 integer matrices (|entries| < 2^76, K < 2^15) as
+    a numpy object array of Python ints: FP64 GEMMs of 19-bit pieces whose
+    partial sums stay below 2^53 (exact in any order), recombined with exact
+    integers."""
     assert Ai.shape[1] < 2 ** 15
-    assert np.abs(Ai).max(initial=0) < 2.0 ** 57 and np.abs(Bi).max(initial=0) < 2.0 ** 57
-    pa, pb = _pieces(Ai), _pieces(Bi)
+    big = max(np.abs(Ai).max(initial=0), np.abs(Bi).max(initial=0))
+    assert big < 2.0 ** 76
+    count = 3 if big < 2.0 ** 57 else 4
+    pa, pb = _pieces(Ai, count), _pieces(Bi, count)
     to_int = np.vectorize(int, otypes=[object])
     C = np.zeros((Ai.shape[0], Bi.shape[1]), dtype=object)
-    for p in range(3):
-        for q in range(3):
+    for p in range(count):
+        for q in range(count):
             G = pa[p] @ pb[q]
             C = C + to_int(G) * (1 << (19 * (p + q)))
     return C

@@ -1,7 +1,10 @@
 """GPU parity of the uneven batch entry point (caps_multiply_batch_device):
 a set of independent problems of different sizes, depths and patterns runs as
-one captured graph, and each result equals the same problem run alone (FP64:
-the oracle's bits; residue-domain mode: the exact-integer model's bits)."""
+one captured graph.  FP64: on integer inputs each result is the oracle's, bit
+for bit; on real inputs it is bit-identical to the same problem run alone
+(the leaf GEMM accumulates in MFMA order, so real-data FP64 results match the
+oracle only to rounding).  Residue-domain mode: the exact-integer model's
+bits on real inputs."""
 import numpy as np
 import pytest
 
@@ -29,12 +32,16 @@ ITEMS = [(64, 2, 1, "bb"), (128, 2, 1, "bd"), (64, 2, 1, "bb"), (56, 3, 1, "bbb"
          (192, 1, 1, "b"), (128, 2, 8, "dd"), (64, 2, 1, "bb")]
 
 
-def _problems(seed):
+def _problems(seed, integers=False):
     rng = np.random.default_rng(seed)
     out = []
     for n, r, bs, pat in ITEMS:
-        A = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-8, 9, (n, n)))
-        B = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-8, 9, (n, n)))
+        if integers:
+            A = rng.integers(-64, 65, (n, n)).astype(np.float64)
+            B = rng.integers(-64, 65, (n, n)).astype(np.float64)
+        else:
+            A = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-8, 9, (n, n)))
+            B = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-8, 9, (n, n)))
         out.append((A, B, n, r, bs, pat))
     return out
 
@@ -53,14 +60,35 @@ def _run_batch(probs, mode, moduli=0, flags=0):
     return [oracle.from_caps(c.cpu().numpy(), p[2], p[3], p[4]) for c, p in zip(Cs, probs)]
 
 
+def _run_single(A, B, n, r, bs, pat, mode=caps_amd.MODE_FP64):
+    d = caps_amd.MatDescriptor.one_device(n, r, bs)
+    dA = torch.from_numpy(oracle.to_caps(A, r, bs)).cuda()
+    dB = torch.from_numpy(oracle.to_caps(B, r, bs)).cuda()
+    dC = torch.empty_like(dA)
+    caps_amd.multiply_device(dA, dB, dC, d, pat, mode)
+    torch.cuda.synchronize()
+    return oracle.from_caps(dC.cpu().numpy(), n, r, bs)
+
+
 @pytest.mark.parametrize("flags", [0, caps_amd.FLAG_NO_GRAPH])
-def test_batch_fp64_matches_oracle(flags):
-    probs = _problems(1)
+def test_batch_fp64_integers_match_oracle(flags):
+    probs = _problems(1, integers=True)
     got = _run_batch(probs, caps_amd.MODE_FP64, 0, flags)
     for (A, B, n, r, bs, pat), g in zip(probs, got):
         want = oracle.from_caps(oracle.multiply(oracle.to_caps(A, r, bs), oracle.to_caps(B, r, bs),
                                                 n, r, bs, pat, 2), n, r, bs)
         assert np.array_equal(g, want), (n, r, bs, pat)
+
+
+def test_batch_fp64_reals_match_single_calls():
+    probs = _problems(4)
+    got = _run_batch(probs, caps_amd.MODE_FP64)
+    for (A, B, n, r, bs, pat), g in zip(probs, got):
+        one = _run_single(A, B, n, r, bs, pat)
+        assert np.array_equal(g, one), (n, r, bs, pat, np.abs(g - one).max())
+        want = oracle.from_caps(oracle.multiply(oracle.to_caps(A, r, bs), oracle.to_caps(B, r, bs),
+                                                n, r, bs, pat, 2), n, r, bs)
+        assert np.abs(g - want).max() <= 1e-13 * np.abs(want).max()
 
 
 def test_batch_crt_matches_model():

@@ -105,6 +105,16 @@ def test_exact_int_matmul_against_python_ints():
     assert (split.exact_int_matmul(A, B) == want).all()
 
 
+def test_exact_int_matmul_wide_entries():
+    """Entries up to 2^63 (16 moduli) take four pieces."""
+    rng = np.random.default_rng(9)
+    A = np.trunc(rng.uniform(-1, 1, (5, 11)) * 2.0 ** 63)
+    B = np.trunc(rng.uniform(-1, 1, (11, 6)) * 2.0 ** 62)
+    want = np.array([[sum(int(A[i, k]) * int(B[k, j]) for k in range(11)) for j in range(6)]
+                     for i in range(5)], dtype=object)
+    assert (split.exact_int_matmul(A, B) == want).all()
+
+
 def test_crt_mode_error_level():
     """On u * 2^e data the exact-product model is within 1e-15 normwise of
     the compensated classical product, far below Winograd's FP64 floor."""
