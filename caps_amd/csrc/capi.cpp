@@ -178,62 +178,13 @@ int check_cutoff(const caps_desc &d, int mode, int cutoff) {
   return CAPS_OK;
 }
 
-int get_plan_locked(State &s, const double *A, const double *B, double *C, const caps_desc &d,
-                    const std::string &pat, int mode, int moduli, int cutoff, bool fuse,
-                    Plan *&out) {
-  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff, fuse, std::string()};
-  for (auto &p : s.plans)
-    if (p->key == key) {
-      out = p.get();
-      p->last_use = ++s.clock;
-      return CAPS_OK;
-    }
-  evict_locked(s, MAX_PLANS - 1);
-  auto plan = std::make_unique<Plan>();
-  plan->key = key;
-  build_plan(*plan, A, B, C, d, pat, mode, moduli, cutoff, nullptr, fuse);
-  const size_t need = plan->work_bytes;
-  if (need) {
-    size_t freeb = 0, totalb = 0;
-    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-    if (need > freeb) {
-      evict_locked(s, 0);
-      HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-      if (need > freeb)
-        return fail(CAPS_ENOMEM, "plan needs %.1f MiB of device workspace, %.1f MiB free",
-                    need / 1048576.0, freeb / 1048576.0);
-    }
-    hipError_t e = hipMalloc(&plan->d_work, need);
-    if (e != hipSuccess)
-      return fail(CAPS_ENOMEM, "hipMalloc(%zu) for workspace: %s", need, hipGetErrorString(e));
-  }
-  build_plan(*plan, A, B, C, d, pat, mode, moduli, cutoff, plan->d_work, fuse);
-  if (!plan->host_tables.empty()) {
-    hipError_t e = hipMalloc(&plan->d_tables, plan->host_tables.size());
-    if (e != hipSuccess) {
-      free_plan(*plan);
-      return fail(CAPS_ENOMEM, "hipMalloc for pointer tables: %s", hipGetErrorString(e));
-    }
-    e = hipMemcpy(plan->d_tables, plan->host_tables.data(), plan->host_tables.size(),
-                  hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      free_plan(*plan);
-      return fail(CAPS_EHIP, "uploading pointer tables: %s", hipGetErrorString(e));
-    }
-  }
-  plan->last_use = ++s.clock;
-  out = plan.get();
-  s.plans.push_back(std::move(plan));
-  return CAPS_OK;
-}
-
 // allocate, build and upload a plan whose builder is `build(plan, work)`
 template <typename F>
 int make_plan_locked(State &s, const PlanKey &key, F build, Plan *&out) {
   evict_locked(s, MAX_PLANS - 1);
   auto plan = std::make_unique<Plan>();
   plan->key = key;
-  build(*plan, nullptr);
+  if (build(*plan, nullptr) != CAPS_OK) return fail(CAPS_EINVAL, "internal: plan layout check failed");
   const size_t need = plan->work_bytes;
   if (need) {
     size_t freeb = 0, totalb = 0;
@@ -249,7 +200,10 @@ int make_plan_locked(State &s, const PlanKey &key, F build, Plan *&out) {
     if (e != hipSuccess)
       return fail(CAPS_ENOMEM, "hipMalloc(%zu) for workspace: %s", need, hipGetErrorString(e));
   }
-  build(*plan, plan->d_work);
+  if (build(*plan, plan->d_work) != CAPS_OK) {
+    free_plan(*plan);
+    return fail(CAPS_EINVAL, "internal: plan layout check failed");
+  }
   if (!plan->host_tables.empty()) {
     hipError_t e = hipMalloc(&plan->d_tables, plan->host_tables.size());
     if (e == hipSuccess)
@@ -264,6 +218,24 @@ int make_plan_locked(State &s, const PlanKey &key, F build, Plan *&out) {
   out = plan.get();
   s.plans.push_back(std::move(plan));
   return CAPS_OK;
+}
+
+int get_plan_locked(State &s, const double *A, const double *B, double *C, const caps_desc &d,
+                    const std::string &pat, int mode, int moduli, int cutoff, bool fuse,
+                    Plan *&out) {
+  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff, fuse, std::string()};
+  for (auto &p : s.plans)
+    if (p->key == key) {
+      out = p.get();
+      p->last_use = ++s.clock;
+      return CAPS_OK;
+    }
+  return make_plan_locked(
+      s, key,
+      [&](Plan &pl, void *work) {
+        return build_plan(pl, A, B, C, d, pat, mode, moduli, cutoff, work, fuse);
+      },
+      out);
 }
 
 int capture_graph(Plan &p) {
@@ -597,7 +569,7 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
     rc = make_plan_locked(
         s, pk,
         [&](Plan &pl, void *work) {
-          build_batch_plan(pl, items, mode, moduli, cutoff, work);
+          return build_batch_plan(pl, items, mode, moduli, cutoff, work);
         },
         plan);
     if (rc) return rc;

@@ -54,6 +54,7 @@ struct Builder {
   bool residue = false;  // matrices are nmod INT8 planes
   int es = 8;            // bytes per element: 8 (FP64) or 1 (residue)
   int planes = 1;        // 1 or nmod
+  bool broken = false;   // an internal layout invariant failed (plan unusable)
 
   size_t alloc(size_t bytes) {
     size_t off = top;
@@ -340,6 +341,12 @@ struct Builder {
     top = mark;
   }
 
+  // the fused FP64 passes move 16-byte pairs
+  static bool aligned16(const Prob &p) {
+    return ((reinterpret_cast<uintptr_t>(p.A.p) | reinterpret_cast<uintptr_t>(p.B.p) |
+             reinterpret_cast<uintptr_t>(p.C.p)) & 15) == 0;
+  }
+
   // Can two 'b' levels starting at `depth` run as fused passes?
   bool fusable(size_t batch, int s, int nrec, int depth) const {
     if (!fuse || batch != 1 || nrec - cutoff < 2) return false;
@@ -363,41 +370,47 @@ struct Builder {
     const int quadA[7] = {0, 2, -1, -1, -1, -1, 3};  // T1 = A11, T2 = A12, T7 = A22
     const int quadB[7] = {0, 1, -1, -1, -1, 3, -1};  // S1 = B11, S2 = B21, S6 = B22
     Mat LA = alloc_mats(40, q), LB = alloc_mats(40, q), P2 = alloc_mats(49, q);
+    // the kernels address the 40 written operands of a side as out + k * step
+    // (k counts the non-plain (i, j) in order) and product (i, j) as
+    // q + (7 i + j) * step
     BBPre pre{};
     pre.in[0] = pb.A.p;
     pre.in[1] = pb.B.p;
     pre.sin[0] = pb.A.ps;
     pre.sin[1] = pb.B.ps;
+    pre.out[0] = LA.p;
+    pre.out[1] = LB.p;
+    pre.sout[0] = LA.ps;
+    pre.sout[1] = LB.ps;
+    pre.step = q * es;
     std::vector<Prob> next(49);
     int ka = 0, kb = 0;
     for (int i = 0; i < 7; i++)
       for (int j = 0; j < 7; j++) {
         const int k = 7 * i + j;
         Mat a, b;
-        if (quadA[i] >= 0 && quadA[j] >= 0) {
+        if (bb_plain(0, i) && bb_plain(0, j)) {
           a = at(pb.A, (4 * quadA[i] + quadA[j]) * q);
         } else {
           a = at(LA, (long long)ka++ * q);
-          pre.out[0][k] = a.p;
-          pre.sout[0][k] = a.ps;
         }
-        if (quadB[i] >= 0 && quadB[j] >= 0) {
+        if (bb_plain(1, i) && bb_plain(1, j)) {
           b = at(pb.B, (4 * quadB[i] + quadB[j]) * q);
         } else {
           b = at(LB, (long long)kb++ * q);
-          pre.out[1][k] = b.p;
-          pre.sout[1][k] = b.ps;
         }
         next[k] = {a, b, at(P2, k * q)};
       }
+    if (ka != 40 || kb != 40) broken = true;
     if (residue) ensure_consts();
     push(STEP_BB_PRE, residue, 1, q, 0, add_table(&pre, sizeof(pre)));
     rec(next, sub, nrec - 2, depth + 2);
     BBPost post{};
-    for (int k = 0; k < 49; k++) {
-      post.q[k] = next[k].C.p;
-      post.sq[k] = next[k].C.ps;
-    }
+    post.q = P2.p;
+    post.step = q * es;
+    post.sq = P2.ps;
+    for (int k = 0; k < 49; k++)
+      if (next[k].C.p != P2.p + k * q * es || next[k].C.ps != P2.ps) broken = true;
     if (!residue) {
       post.c = reinterpret_cast<double *>(pb.C.p);
     } else {
@@ -418,7 +431,7 @@ struct Builder {
       leaf(batch, s);
       return;
     }
-    if (!residue && fusable(batch.size(), s, nrec, depth)) {
+    if (!residue && fusable(batch.size(), s, nrec, depth) && aligned16(batch[0])) {
       bfs2(batch[0], s, nrec, depth, nullptr);
       return;
     }
@@ -574,7 +587,7 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
     b.rec(top, d.lda, d.nrec, 0);
   }
   plan.work_bytes = b.peak;
-  return CAPS_OK;
+  return b.broken ? CAPS_EINVAL : CAPS_OK;
 }
 
 int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, int moduli,
@@ -620,6 +633,7 @@ int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, 
       b.rec(batch, g[0]->desc.lda, g[0]->desc.nrec, 0);
     }
     peak = std::max(peak, b.peak);
+    if (b.broken) return CAPS_EINVAL;
   }
   plan.work_bytes = peak;
   plan.exec_pattern = "BATCH,";

@@ -8,20 +8,29 @@
 
 namespace caps {
 
+// Level-1 operands that are plain quadrants (passed by pointer, not
+// written): T1 = A11, T2 = A12, T7 = A22 and S1 = B11, S2 = B21, S6 = B22.
+// Depth-2 operand (i, j) is plain iff i and j both are.
+__host__ __device__ constexpr bool bb_plain(int side, int t) {
+  return side == 0 ? (t == 0 || t == 1 || t == 6) : (t == 0 || t == 1 || t == 5);
+}
+
 // pre-additions of two 'b' levels for both operands of one sub-problem
 struct BBPre {
   const void *in[2];      // A (side 0) and B (side 1): 16 depth-2 sub-blocks of q elements,
                           // sub-block (a, b) at element (4a + b) * q
-  void *out[2][49];       // depth-2 operand (i, j) = operand j of level-1 operand i;
-                          // null: a plain sub-block passed by pointer
+  void *out[2];           // the 40 written depth-2 operands (i, j) = operand j of level-1
+                          // operand i, not plain, in (i, j) order, one every `step` bytes
+  long long step;         // bytes between consecutive operands (q elements)
   long long sin[2];       // residue mode: plane strides (bytes)
-  long long sout[2][49];
+  long long sout[2];
 };
 
 // post-combinations of two 'b' levels for one sub-problem
 struct BBPost {
-  const void *q[49];      // depth-2 products (i, j)
-  long long sq[49];       // residue mode: plane strides
+  const void *q;          // the 49 depth-2 products (i, j), product 7 i + j at (7 i + j) * step
+  long long step;         // bytes
+  long long sq;           // residue mode: plane stride
   double *c;              // FP64 mode: C's 16 depth-2 sub-blocks at (4a + b) * q
   // residue mode: reconstruct into FP64 C (CAPS layout) directly
   double *out_c;

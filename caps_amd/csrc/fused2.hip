@@ -45,182 +45,231 @@ __device__ __forceinline__ void level_ops(int side, const V (&q)[4], V (&o)[7]) 
 }
 
 // ---------------------------------------------------------------- FP64 pre
-// thread: 2 consecutive positions (16-byte accesses); grid.y: side
+// thread: 2 consecutive positions, 16-byte loads and stores; grid.y: side
 __global__ __launch_bounds__(T) void bb_pre_f64_kernel(const BBPre *__restrict__ P, long long q) {
   const int side = blockIdx.y;
   const double *in = reinterpret_cast<const double *>(P->in[side]);
   const long long n2 = q >> 1;
   for (long long u = (long long)blockIdx.x * T + threadIdx.x; u < n2;
        u += (long long)gridDim.x * T) {
-    double x0[4][4], x1[4][4];  // [q1][q2], two positions
+    double l1[2][7][4];  // [position][level-1 operand i][sub-position q2]
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) {
+      double q0[4], q1[4];
 #pragma unroll
-      for (int b = 0; b < 4; b++) {
+      for (int a = 0; a < 4; a++) {
         const double2 v = reinterpret_cast<const double2 *>(in + (4 * a + b) * q)[u];
-        x0[a][b] = v.x;
-        x1[a][b] = v.y;
+        q0[a] = v.x;
+        q1[a] = v.y;
       }
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      double (&x)[4][4] = h ? x1 : x0;
-      double l1[7][4];  // level-1 operand i at sub-position q2
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        double qv[4] = {x[0][b], x[1][b], x[2][b], x[3][b]}, o[7];
-        level_ops(side, qv, o);
-#pragma unroll
-        for (int i = 0; i < 7; i++) l1[i][b] = o[i];
-      }
+      double o0[7], o1[7];
+      level_ops(side, q0, o0);
+      level_ops(side, q1, o1);
 #pragma unroll
       for (int i = 0; i < 7; i++) {
-        double qv[4] = {l1[i][0], l1[i][1], l1[i][2], l1[i][3]}, o[7];
-        level_ops(side, qv, o);
+        l1[0][i][b] = o0[i];
+        l1[1][i][b] = o1[i];
+      }
+    }
+    char *out = reinterpret_cast<char *>(P->out[side]);
+    int k = 0;  // index among the written operands (compile-time after unrolling)
 #pragma unroll
-        for (int j = 0; j < 7; j++) {
-          double *dst = reinterpret_cast<double *>(P->out[side][7 * i + j]);
-          if (dst) dst[2 * u + h] = o[j];
-        }
+    for (int i = 0; i < 7; i++) {
+      double q0[4] = {l1[0][i][0], l1[0][i][1], l1[0][i][2], l1[0][i][3]};
+      double q1[4] = {l1[1][i][0], l1[1][i][1], l1[1][i][2], l1[1][i][3]};
+      double o0[7], o1[7];
+      level_ops(side, q0, o0);
+      level_ops(side, q1, o1);
+#pragma unroll
+      for (int j = 0; j < 7; j++) {
+        if (bb_plain(side, i) && bb_plain(side, j)) continue;
+        reinterpret_cast<double2 *>(out + k * P->step)[u] = make_double2(o0[j], o1[j]);
+        k++;
       }
     }
   }
 }
 
 // ---------------------------------------------------------------- FP64 post
-// thread: 2 consecutive positions
+// thread: 2 consecutive positions, 16-byte loads and stores
 __global__ __launch_bounds__(T) void bb_post_f64_kernel(const BBPost *__restrict__ P, long long q) {
   const long long n2 = q >> 1;
   for (long long u = (long long)blockIdx.x * T + threadIdx.x; u < n2;
        u += (long long)gridDim.x * T) {
+    double l1[2][7][4];  // [position][product i of level 1][sub-position q2]
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      double l1[7][4];  // product i of level 1 at sub-position q2
+    for (int i = 0; i < 7; i++) {
+      double q0[7], q1[7], c0[4], c1[4];
 #pragma unroll
-      for (int i = 0; i < 7; i++) {
-        double qv[7], c[4];
-#pragma unroll
-        for (int j = 0; j < 7; j++)
-          qv[j] = reinterpret_cast<const double *>(P->q[7 * i + j])[2 * u + h];
-        Ew<EW_BFS_POST>::apply(qv, c);  // C11 C21 C12 C22 of product i
-#pragma unroll
-        for (int b = 0; b < 4; b++) l1[i][b] = c[b];
+      for (int j = 0; j < 7; j++) {
+        const double2 v =
+            reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(P->q) +
+                                              (7 * i + j) * P->step)[u];
+        q0[j] = v.x;
+        q1[j] = v.y;
       }
+      Ew<EW_BFS_POST>::apply(q0, c0);  // C11 C21 C12 C22 of product i
+      Ew<EW_BFS_POST>::apply(q1, c1);
 #pragma unroll
       for (int b = 0; b < 4; b++) {
-        double qv[7], c[4];
-#pragma unroll
-        for (int i = 0; i < 7; i++) qv[i] = l1[i][b];
-        Ew<EW_BFS_POST>::apply(qv, c);
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-          reinterpret_cast<double *>(P->c)[(4 * a + b) * q + 2 * u + h] = c[a];
+        l1[0][i][b] = c0[b];
+        l1[1][i][b] = c1[b];
       }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      double q0[7], q1[7], c0[4], c1[4];
+#pragma unroll
+      for (int i = 0; i < 7; i++) {
+        q0[i] = l1[0][i][b];
+        q1[i] = l1[1][i][b];
+      }
+      Ew<EW_BFS_POST>::apply(q0, c0);
+      Ew<EW_BFS_POST>::apply(q1, c1);
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+        reinterpret_cast<double2 *>(reinterpret_cast<double *>(P->c) + (4 * a + b) * q)[u] =
+            make_double2(c0[a], c1[a]);
     }
   }
 }
 
 // ---------------------------------------------------------------- residue pre
-// thread: 4 consecutive positions (4-byte accesses); grid.y: side; grid.z: modulus
+// thread: 4 consecutive positions (4-byte accesses); grid.y: side; grid.z:
+// modulus.  The 7 depth-2 operands of one level-1 operand are finished and
+// stored before the next, so only the 16 input words and 7 output words stay
+// live.  Depth-2 values are sums of at most 16 residues: |x| <= 2048.
 __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__ P, long long q,
                                                        const OzakiConsts *__restrict__ kc) {
   const int side = blockIdx.y, l = blockIdx.z;
   const int mi = kc->m[l];
-  const float inv = 1.0f / (float)mi;
+  const float mf = (float)mi, inv = 1.0f / (float)mi;
   const int8_t *in = reinterpret_cast<const int8_t *>(P->in[side]) + l * P->sin[side];
+  char *obase = reinterpret_cast<char *>(P->out[side]) + l * P->sout[side];
   const long long n4 = q >> 2;
   for (long long u = (long long)blockIdx.x * T + threadIdx.x; u < n4;
        u += (long long)gridDim.x * T) {
+    int k = 0;  // index among the written operands (compile-time after unrolling)
     unsigned w[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
       for (int b = 0; b < 4; b++) w[a][b] = reinterpret_cast<const unsigned *>(in + (4 * a + b) * q)[u];
-    unsigned out[49];
 #pragma unroll
-    for (int k = 0; k < 49; k++) out[k] = 0;
+    for (int i = 0; i < 7; i++) {
+      // recompute from the 16 input words for each i rather than keep the
+      // 112 level-1 values of all i live (an empty asm hides the reuse)
 #pragma unroll
-    for (int byte = 0; byte < 4; byte++) {
-      int l1[7][4];
+      for (int a = 0; a < 4; a++)
 #pragma unroll
-      for (int b = 0; b < 4; b++) {
-        int qv[4], o[7];
+        for (int b = 0; b < 4; b++) asm volatile("" : "+v"(w[a][b]));
+      unsigned out[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int a = 0; a < 4; a++) qv[a] = (int)(int8_t)((w[a][b] >> (8 * byte)) & 0xff);
-        level_ops(side, qv, o);
+      for (int byte = 0; byte < 4; byte++) {
+        int l1[4];  // level-1 operand i at the four sub-positions
 #pragma unroll
-        for (int i = 0; i < 7; i++) l1[i][b] = o[i];
-      }
+        for (int b = 0; b < 4; b++) {
+          int qv[4], o[7];
 #pragma unroll
-      for (int i = 0; i < 7; i++) {
-        int qv[4] = {l1[i][0], l1[i][1], l1[i][2], l1[i][3]}, o[7];
-        level_ops(side, qv, o);
+          for (int a = 0; a < 4; a++) qv[a] = __builtin_amdgcn_sbfe(w[a][b], 8 * byte, 8);
+          level_ops(side, qv, o);
+          l1[b] = o[i];
+        }
+        int o[7];
+        level_ops(side, l1, o);
 #pragma unroll
         for (int j = 0; j < 7; j++)
-          out[7 * i + j] |= (unsigned)(mod_i32(o[j], mi, inv) & 0xff) << (8 * byte);
+          out[j] |= (unsigned)(mod_small(o[j], mf, inv) & 0xff) << (8 * byte);
       }
-    }
 #pragma unroll
-    for (int k = 0; k < 49; k++) {
-      int8_t *dst = reinterpret_cast<int8_t *>(P->out[side][k]);
-      if (dst) reinterpret_cast<unsigned *>(dst + l * P->sout[side][k])[u] = out[k];
+      for (int j = 0; j < 7; j++) {
+        if (bb_plain(side, i) && bb_plain(side, j)) continue;
+        reinterpret_cast<unsigned *>(obase + k * P->step)[u] = out[j];
+        k++;
+      }
     }
   }
 }
 
 // ---------------------------------------------------------------- residue post + CRT
-// thread: one position; loops over the moduli, accumulating the CRT sums of
-// C's 16 sub-blocks, then reconstructs, unscales and writes FP64 C
+// One block per PPOS consecutive positions of the depth-2 sub-blocks, two
+// phases through LDS:
+//   1. lane per (4 positions, modulus): 49 product words, both levels of the
+//      post-combination, the 16 residues of C's sub-blocks into LDS;
+//   2. lane per (position, sub-block of C), consecutive lanes on consecutive
+//      positions: CRT over the moduli from LDS, unscale, FP64 store.
+constexpr int PPOS = 128;
+constexpr int PG = PPOS / 4;  // 4-position words per block
+
 template <int NP>
 __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict__ P, long long q,
                                                         const OzakiConsts *__restrict__ kc) {
-  const int m0 = P->m0, sub = P->sub, nrec2 = P->nrec2, bs = P->bs;
-  const long long mm0 = (long long)m0 * m0;
-  for (long long u = (long long)blockIdx.x * T + threadIdx.x; u < q;
-       u += (long long)gridDim.x * T) {
-    double S[16][NP];
+  __shared__ unsigned R[OZ_MAX_MODULI][16][PG];
+  const int nmod = kc->nmod;
+  const long long pos0 = (long long)blockIdx.x * PPOS;
+  for (int unit = threadIdx.x; unit < nmod * PG; unit += T) {
+    const int g = unit % PG, l = unit / PG;
+    const long long u4 = pos0 / 4 + g;
+    if (4 * u4 >= q) continue;  // q % 4 == 0
+    const int mi = kc->m[l];
+    const float mf = (float)mi, inv = 1.0f / mf;
+    const char *qb = reinterpret_cast<const char *>(P->q) + l * P->sq;
+    unsigned w[49];
 #pragma unroll
-    for (int c = 0; c < 16; c++)
+    for (int k = 0; k < 49; k++) w[k] = reinterpret_cast<const unsigned *>(qb + k * P->step)[u4];
+    unsigned out[16];
 #pragma unroll
-      for (int p = 0; p < NP; p++) S[c][p] = 0.0;
-    for (int l = 0; l < kc->nmod; l++) {
-      const int mi = kc->m[l];
-      const float inv = 1.0f / (float)mi;
+    for (int o = 0; o < 16; o++) out[o] = 0;
+#pragma unroll
+    for (int byte = 0; byte < 4; byte++) {
       int l1[7][4];
 #pragma unroll
       for (int i = 0; i < 7; i++) {
         int qv[7], c[4];
 #pragma unroll
-        for (int j = 0; j < 7; j++)
-          qv[j] = (int)reinterpret_cast<const int8_t *>(P->q[7 * i + j])[l * P->sq[7 * i + j] + u];
+        for (int j = 0; j < 7; j++) qv[j] = __builtin_amdgcn_sbfe(w[7 * i + j], 8 * byte, 8);
         Ew<EW_BFS_POST>::apply(qv, c);
 #pragma unroll
-        for (int b = 0; b < 4; b++) l1[i][b] = c[b];  // |.| <= 512
+        for (int bb = 0; bb < 4; bb++) l1[i][bb] = c[bb];  // |.| <= 512
       }
 #pragma unroll
-      for (int b = 0; b < 4; b++) {
+      for (int bb = 0; bb < 4; bb++) {
         int qv[7], c[4];
 #pragma unroll
-        for (int i = 0; i < 7; i++) qv[i] = l1[i][b];
+        for (int i = 0; i < 7; i++) qv[i] = l1[i][bb];
         Ew<EW_BFS_POST>::apply(qv, c);  // |.| <= 2048
 #pragma unroll
-        for (int a = 0; a < 4; a++) crt_acc<NP>(S[4 * a + b], mod_i32(c[a], mi, inv), l, kc);
+        for (int aa = 0; aa < 4; aa++)
+          out[4 * aa + bb] |= (unsigned)(mod_small(c[aa], mf, inv) & 0xff) << (8 * byte);
       }
     }
+#pragma unroll
+    for (int o = 0; o < 16; o++) R[l][o][g] = out[o];
+  }
+  __syncthreads();
+
+  const int m0 = P->m0, sub = P->sub, nrec2 = P->nrec2, bs = P->bs;
+  const long long mm0 = (long long)m0 * m0;
+  for (int pr = threadIdx.x; pr < 16 * PPOS; pr += T) {
+    const int o = pr / PPOS, pl = pr % PPOS;
+    const long long u = pos0 + pl;
+    if (u >= q) continue;
+    double S[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) S[p] = 0.0;
+    for (int l = 0; l < nmod; l++)
+      crt_acc<NP>(S, __builtin_amdgcn_sbfe(R[l][o][pl >> 2], 8 * (pl & 3), 8), l, kc);
     // position u of a depth-2 sub-block: leaf L2 (quadtree of nrec2 levels), column-major leaf
     const long long L2 = u / mm0, e = u - L2 * mm0;
     const int li = (int)(e % m0), lj = (int)(e / m0);
     int bi, bj;
     leaf_coords((int)L2, nrec2, bi, bj);
-    const long long dst_in_sub = L2 * mm0 + leaf_elem_offset(m0, bs, li, lj);
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        const int gi = (a & 1) * 2 * sub + (b & 1) * sub + bi * m0 + li;
-        const int gj = (a >> 1) * 2 * sub + (b >> 1) * sub + bj * m0 + lj;
-        const double X = crt_finish<NP>(S[4 * a + b], kc);
-        P->out_c[(4 * a + b) * q + dst_in_sub] = ldexp(X, -(P->sa[gi] + P->sb[gj]));
-      }
+    const int aa = o >> 2, bb = o & 3;
+    const int gi = (aa & 1) * 2 * sub + (bb & 1) * sub + bi * m0 + li;
+    const int gj = (aa >> 1) * 2 * sub + (bb >> 1) * sub + bj * m0 + lj;
+    const double X = crt_finish<NP>(S, kc);
+    P->out_c[o * q + L2 * mm0 + leaf_elem_offset(m0, bs, li, lj)] =
+        ldexp(X, -(P->sa[gi] + P->sb[gj]));
   }
 }
 
@@ -254,7 +303,8 @@ hipError_t launch_bb_post(const BBPost *P, long long q, bool residue, int npiece
     if (q % 2) return hipErrorInvalidValue;
     hipLaunchKernelGGL(bb_post_f64_kernel, dim3(blocks_for(q / 2, 1)), dim3(T), 0, s, P, q);
   } else {
-    dim3 g(blocks_for(q, 1));
+    if (q % 4) return hipErrorInvalidValue;
+    dim3 g((unsigned)((q + PPOS - 1) / PPOS));
     switch (npieces) {
       case 1:
       case 2: hipLaunchKernelGGL((bb_post_crt_kernel<2>), g, dim3(T), 0, s, P, q, kc); break;

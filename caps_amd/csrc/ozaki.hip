@@ -327,6 +327,129 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
   }
 }
 
+// ------------------------------------------------------------------ INT8 GEMM, large tiles
+// 256 x 256 block tiles, 8 waves (2 per SIMD) of 128 x 64 (32 accumulator
+// tiles of 16 x 16 per wave), K-steps of 128.  Operands go global -> LDS
+// directly (global_load_lds_dwordx4: each lane's 16 bytes land at the wave's
+// LDS base + 16 * lane), double-buffered, so no registers hold the next
+// stage.  Per MAC this reads 3/4 of the LDS bytes and half the L2 bytes of
+// the 128 x 128 kernel above.  Needs m0 % 256 == 0.
+//
+// LDS stage layout per operand: 256 rows x 8 chunks of 16 bytes (k), no
+// padding; chunk c of row r sits in slot c ^ swz(r), swz(r) = (r & 7) ^
+// ((r >> 3) & 1), so the b128 reads of 16 lanes (16 consecutive rows, one
+// chunk) touch 16 distinct 16-byte bank groups.
+constexpr int JBM = 256, JBK = 128, JT = 512;
+constexpr int JSTAGE = JBM * JBK;  // 32 KB per operand per stage
+
+__device__ __forceinline__ int jswz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+
+typedef __attribute__((address_space(3))) void *lds_ptr;
+
+// one stage: wave w loads rows 8 (8 j + w) .. + 7 of A and of B, 8 lanes per
+// 128-byte row; swz(r) = (lane >> 3) ^ (w & 1) for those rows
+__device__ __forceinline__ void stage_load(const int8_t *__restrict__ Ab,
+                                           const int8_t *__restrict__ Bb, long long ld,
+                                           int8_t *As, int8_t *Bs, int wave, int lane) {
+  const int rl = lane >> 3;
+  const int c = (lane & 7) ^ rl ^ (wave & 1);  // chunk this lane fetches (same for every j)
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int r = j * 64 + wave * 8 + rl;
+    const int slot0 = (j * 8 + wave) * 64;  // first 16-byte slot of this wave's 1 KB
+    __builtin_amdgcn_global_load_lds(Ab + r * ld + c * 16, (lds_ptr)(As + slot0 * 16), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(Bb + r * ld + c * 16, (lds_ptr)(Bs + slot0 * 16), 16, 0, 0);
+  }
+}
+
+__global__ __launch_bounds__(JT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
+                    const OzakiConsts *__restrict__ kc, TileMap map, int m0, int d) {
+  __shared__ __attribute__((aligned(1024))) int8_t A0[JSTAGE];
+  __shared__ __attribute__((aligned(1024))) int8_t A1[JSTAGE];
+  __shared__ __attribute__((aligned(1024))) int8_t B0[JSTAGE];
+  __shared__ __attribute__((aligned(1024))) int8_t B1[JSTAGE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves: rows wm * 128, columns wn * 64
+  const int l16 = lane & 15, lk = lane >> 4;
+  const int G = gridDim.x;
+  const int nk = m / JBK;
+  // read offsets (bytes) of this lane's row l16, chunk lk (kh = 0) and lk + 4 (kh = 1)
+  const int s16 = jswz(l16);
+  const int rd0 = l16 * JBK + ((lk ^ s16) << 4);
+  const int rd1 = l16 * JBK + (((lk + 4) ^ s16) << 4);
+  const int aoff = wm * 128 * JBK, boff = wn * 64 * JBK;
+
+  for (int round = 0; round * G < map.total; round++) {
+    const int tile = round * G + xcd_tile(blockIdx.x, G);
+    if (tile >= map.total) break;
+    int prob, ti, tj;
+    tile_coords(map, tile, prob, ti, tj);
+    const int leaf = prob / nmod, l = prob - leaf * nmod;
+    const int8_t *__restrict__ Ap = tab[leaf].a + l * tab[leaf].sa;  // row-major leaves
+    const int8_t *__restrict__ Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major leaves
+    int8_t *__restrict__ Cp = tab[leaf].c + l * tab[leaf].sc;
+    const int mi = kc->m[l];
+    const float invf = 1.0f / (float)mi;
+    const int row0 = ti * JBM, col0 = tj * JBM;
+
+    v4i acc[8][4];
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[r][c] = v4i{0, 0, 0, 0};
+
+    const long long mm0 = (long long)m0 * m0;
+    const long long arow = (long long)(row0 % m0) * m0, bcol = (long long)(col0 % m0) * m0;
+    // (the previous tile's k loop ended with a barrier after its last LDS reads)
+    stage_load(Ap + quad_leaf(row0 / m0, 0, d) * mm0 + arow,
+               Bp + quad_leaf(0, col0 / m0, d) * mm0 + bcol, m0, A0, B0, wave, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+      const bool odd = kt & 1;
+      if (kt + 1 < nk) {
+        const int k1 = (kt + 1) * JBK;
+        stage_load(Ap + quad_leaf(row0 / m0, k1 / m0, d) * mm0 + arow + (k1 % m0),
+                   Bp + quad_leaf(k1 / m0, col0 / m0, d) * mm0 + bcol + (k1 % m0), m0,
+                   odd ? A0 : A1, odd ? B0 : B1, wave, lane);
+      }
+      const int8_t *Ab = (odd ? A1 : A0) + aoff;
+      const int8_t *Bb = (odd ? B1 : B0) + boff;
+#pragma unroll
+      for (int kh = 0; kh < 2; kh++) {
+        const int rd = kh ? rd1 : rd0;
+        v4i a[8], b[4];
+#pragma unroll
+        for (int r = 0; r < 8; r++) a[r] = *reinterpret_cast<const v4i *>(Ab + rd + r * 16 * JBK);
+#pragma unroll
+        for (int c = 0; c < 4; c++) b[c] = *reinterpret_cast<const v4i *>(Bb + rd + c * 16 * JBK);
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], b[c], acc[r][c], 0, 0, 0);
+      }
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+    }
+
+    // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v
+    int8_t *Cl = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0;
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int row = (row0 % m0) + wm * 128 + r * 16 + 4 * lk;
+        const int col = (col0 % m0) + wn * 64 + c * 16 + l16;
+        unsigned v = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
+        *reinterpret_cast<unsigned *>(Cl + (long long)col * m0 + row) = v;
+      }
+  }
+}
+
 // ------------------------------------------------------------------ reconstruct
 template <int NP, bool VEC>
 __global__ __launch_bounds__(T256) void oz_recon_kernel(const OzLeaf *__restrict__ tab, int m,
@@ -391,11 +514,20 @@ hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, i
                                const OzakiConsts *kc, hipStream_t s) {
   if (count <= 0 || m <= 0) return hipSuccess;
   if (nmod < 2 || nmod > OZ_MAX_MODULI) return hipErrorInvalidValue;
-  bool ok;
-  TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);
-  if (!ok) return hipErrorInvalidValue;
   const int m0 = m >> levels;
   if (levels < 0 || (m0 << levels) != m) return hipErrorInvalidValue;
+  bool ok;
+  if (m0 % JBM == 0) {
+    // large tiles: one block (8 waves, 128 KB of LDS) per CU
+    TileMap map = make_tilemap(m, JBM, (long long)count * nmod, ok);
+    if (!ok) return hipErrorInvalidValue;
+    const int G = persistent_grid(1, device_cu_count(), map.total);
+    hipLaunchKernelGGL(oz_igemm_big_kernel, dim3(G), dim3(JT), 0, s, tab, m, nmod, kc, map, m0,
+                       levels);
+    return hipGetLastError();
+  }
+  TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);
+  if (!ok) return hipErrorInvalidValue;
   const int G = persistent_grid(2, device_cu_count(), map.total);
   // the fast path reads 16-byte rows: leaves of a multiple of 128
   const bool fast = (m0 % IBM == 0);

@@ -59,6 +59,15 @@ __device__ __forceinline__ int sym_residue(double x, int mi, double inv) {
   return ri;
 }
 
+// residue of a small integer (|x| <= 2^20) modulo mi in [-mi/2, mi/2]: x / mi
+// is never a half-integer for odd mi (2x = odd * mi is impossible), and for
+// mi = 256 a tie gives +-128, the same INT8 value after the wrap.  Exact:
+// every step is on integers below 2^24 in FP32.
+__device__ __forceinline__ int mod_small(int x, float mf, float inv) {
+  const float xf = (float)x;
+  return (int)fmaf(-rintf(xf * inv), mf, xf);
+}
+
 // symmetric residue of an INT32 product sum (|x| < 2^31) modulo mi
 __device__ __forceinline__ int mod_i32(int x, int mi, float inv) {
   int q = __float2int_rn((float)x * inv);

@@ -57,6 +57,7 @@ def _setup():
     (128, 2, 8, ["bb", "dh"], 16),     # bs-blocked leaves
     (56, 3, 1, ["bbb", "ddd"], 12),    # odd leaves (m = 7)
     (256, 1, 1, ["b"], 14),            # 128 x 128 leaves: the fast GEMM path
+    (512, 1, 1, ["b", "d"], 14),       # 256 x 256 leaves: the large-tile GEMM
 ])
 def test_crt_mode_bit_exact_vs_model(n, r, bs, patterns, moduli):
     A, B = realdata(n, 100 + n)
@@ -134,6 +135,24 @@ def test_crt_mode_cutoff_bit_exact_vs_model():
         caps_amd.multiply_device(dA, dB, dC, d, pat, mode=CRT, moduli=14, cutoff=cutoff)
         torch.cuda.synchronize()
         assert np.array_equal(oracle.from_caps(dC.cpu().numpy(), n, nrec, 1), want), cutoff
+
+
+def test_crt_large_tile_gemm_bit_exact_vs_model():
+    """Leaves of 256 and quadtree operands of 512 (cutoff) take the 256 x 256
+    tile residue GEMM: same bits as the exact-integer model."""
+    n, nrec = 1024, 2
+    A, B = realdata(n, 41)
+    want = split.crt_mode(A, B, 14)
+    d = caps_amd.MatDescriptor.one_device(n, nrec, 1)
+    dA = torch.from_numpy(oracle.to_caps(A, nrec, 1)).cuda()
+    dB = torch.from_numpy(oracle.to_caps(B, nrec, 1)).cuda()
+    for cutoff, pat in ((0, "bb"), (0, "db"), (1, "b"), (1, "d")):
+        dC = torch.empty_like(dA)
+        print(f"[gpu] crt n=1024 cutoff={cutoff} pattern={pat}", flush=True)
+        caps_amd.multiply_device(dA, dB, dC, d, pat, mode=CRT, moduli=14, cutoff=cutoff)
+        torch.cuda.synchronize()
+        got = oracle.from_caps(dC.cpu().numpy(), n, nrec, 1)
+        assert np.array_equal(got, want), (cutoff, pat, np.abs(got - want).max())
 
 
 @pytest.mark.parametrize("n,nrec,pattern", [(512, 2, "bb"), (1024, 3, "bbd"), (256, 3, "bbb")])
