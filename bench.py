@@ -115,8 +115,115 @@ def cpu_baseline(oracle, n=4096, r=2, threads=None):
                       f"from drand48, {threads} OpenMP threads, {t:.2f} s"}
 
 
+def batch_sizes(count=512, seed=2024):
+    """n = 64 round(exp(U[ln 128, ln 4096]) / 64), clipped to [128, 4096]."""
+    rng = np.random.default_rng(seed)
+    n = 64 * np.round(np.exp(rng.uniform(np.log(128), np.log(4096), count)) / 64)
+    return [int(x) for x in np.clip(n, 128, 4096)]
+
+
+def tune_one_size(torch, caps_amd, n, mode, moduli, reps=2):
+    """Fastest (nrec, pattern, cutoff) for one size: layout depths with
+    leaves >= 32, every candidate the library enumerates for the mode at the
+    given modulus count, timed on the device (caps_measure)."""
+    mask = {caps_amd.MODE_FP64: caps_amd.MASK_FP64,
+            caps_amd.MODE_CRT_INT8: caps_amd.MASK_CRT_INT8}[mode]
+    best = None
+    for r in range(0, 7):
+        if n % (1 << r) or (n >> r) < 32:
+            continue
+        desc = caps_amd.MatDescriptor.one_device(n, r, 1)
+        A = torch.rand(n * n, device="cuda", dtype=torch.float64) * 2 - 1
+        B = torch.rand(n * n, device="cuda", dtype=torch.float64) * 2 - 1
+        C = torch.empty_like(A)
+        for c in caps_amd.candidates(desc, mask):
+            if mode != caps_amd.MODE_FP64 and c.moduli != moduli:
+                continue
+            caps_amd.measure(A, B, C, desc, c, None, reps)
+            if best is None or c.ms < best[3]:
+                best = (r, c.pattern.decode(), c.cutoff, c.ms)
+    return best
+
+
+def batch_workload(args, torch, caps_amd):
+    """512 independent square problems, n log-uniform in [128, 4096] (multiples
+    of 64), entries uniform [-1, 1); per-size depth / pattern / cutoff from
+    timing every candidate; all 512 in one batched plan replayed as one graph."""
+    sizes = batch_sizes()
+    distinct = sorted(set(sizes))
+    eff = sum(2.0 * n ** 3 for n in sizes)
+    K = args.moduli
+    out = {}
+    for mode, name in ((caps_amd.MODE_FP64, "fp64"), (caps_amd.MODE_CRT_INT8, "split")):
+        log(f"[bench] batch: tuning {len(distinct)} sizes ({name})")
+        t0 = time.perf_counter()
+        choice = {n: tune_one_size(torch, caps_amd, n, mode, K) for n in distinct}
+        tune_s = time.perf_counter() - t0
+        caps_amd.release()
+        g = torch.Generator(device="cuda")
+        g.manual_seed(77)
+        As, Bs, Cs, descs, pats, cuts = [], [], [], [], [], []
+        for n in sizes:
+            r, pat, cut, _ = choice[n]
+            As.append(torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1)
+            Bs.append(torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1)
+            Cs.append(torch.empty_like(As[-1]))
+            descs.append(caps_amd.MatDescriptor.one_device(n, r, 1))
+            pats.append(pat)
+            cuts.append(cut)
+        moduli = 0 if mode == caps_amd.MODE_FP64 else K
+
+        def run_batch():
+            caps_amd.multiply_batch_device(As, Bs, Cs, descs, pats, mode, moduli, cutoffs=cuts)
+
+        def run_each():
+            for a, b, c, d, p, k in zip(As, Bs, Cs, descs, pats, cuts):
+                caps_amd.multiply_device(a, b, c, d, p, mode=mode, moduli=moduli, cutoff=k)
+
+        log(f"[bench] batch: timing ({name})")
+        t_batch = time_steps(torch, run_batch, args.steps, args.warmup)
+        t_each = time_steps(torch, run_each, max(2, args.steps // 2), 1)
+        run_batch()
+        torch.cuda.synchronize()
+        log(f"[bench] batch {name}: {t_batch * 1e3:.2f} ms (one call per problem: "
+            f"{t_each * 1e3:.2f} ms); errors vs the 16-modulus reference product")
+        worst = 0.0
+        for a, b, c, d in zip(As, Bs, Cs, descs):
+            ref = torch.empty_like(c)
+            caps_amd.reference_product(a, b, ref, d)
+            worst = max(worst, caps_amd.normwise_error(c, ref, d))
+        out[name] = {"ms": t_batch * 1e3, "gflops": eff / t_batch / 1e9,
+                     "ms_one_call_per_problem": t_each * 1e3,
+                     "gflops_one_call_per_problem": eff / t_each / 1e9,
+                     "max_error": worst, "tuning_s": tune_s,
+                     "choices": {str(n): list(choice[n][:3]) for n in distinct}}
+        del As, Bs, Cs
+        caps_amd.release()
+        torch.cuda.empty_cache()
+    result = {
+        "metric": "effective GFLOP/s (sum of 2n^3 / t), 512 uneven problems n in [128, 4096]",
+        "value": round(out["split"]["gflops"], 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(out["split"]["ms"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 in/out, int8 MFMA leaf products",
+        "data": "synthetic: 512 sizes n = 64 round(exp(U[ln128, ln4096])/64), seed 2024; "
+                "entries uniform [-1, 1)",
+        "config": {"workload": "512 independent multiplies in one batched plan / graph",
+                   "sum_2n3": eff, "moduli": K,
+                   "fp64": {k: v for k, v in out["fp64"].items()},
+                   "split": {k: v for k, v in out["split"].items()}},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=("split", "batch"), default="split",
+                    help="split: the n=8192 headline; batch: 512 uneven problems")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
@@ -144,6 +251,9 @@ def main():
         build.build()
     torch.cuda.set_device(0)
     caps_amd.load()
+    if args.workload == "batch":
+        batch_workload(args, torch, caps_amd)
+        return
     n, r, pat, K = args.n, args.nrec, args.pattern, args.moduli
     desc = caps_amd.MatDescriptor.one_device(n, r, 1)
     eff = 2.0 * n ** 3

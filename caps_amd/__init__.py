@@ -108,7 +108,7 @@ SIGNATURES = {
     "caps_num_entries": (ctypes.c_longlong, [_D]),
     "caps_multiply": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_char_p, _O]),
     "caps_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_char_p, _O, _vp]),
-    "caps_multiply_batch_device": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _O,
+    "caps_multiply_batch_device": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                                   _vp]),
     "caps_to_layout": (ctypes.c_int, [_vp, _vp, _D]),
     "caps_from_layout": (ctypes.c_int, [_vp, _vp, _D]),
@@ -241,9 +241,10 @@ def multiply_device(A, B, C, desc: MatDescriptor, pattern: str | None = None,
 
 
 def multiply_batch_device(As, Bs, Cs, descs, patterns=None, mode: int = MODE_FP64,
-                          moduli: int = 0, flags: int = 0, stream=None) -> None:
+                          moduli: int = 0, flags: int = 0, stream=None, cutoffs=None) -> None:
     """Enqueue independent multiplies C_i = A_i * B_i (device tensors, CAPS
-    layout, per-item descriptors) as one batched plan replayed as one graph."""
+    layout, per-item descriptors, patterns and cutoffs) as one batched plan
+    replayed as one graph."""
     k = len(As)
     assert len(Bs) == k and len(Cs) == k and len(descs) == k
     P = ctypes.c_void_p * k
@@ -255,9 +256,9 @@ def multiply_batch_device(As, Bs, Cs, descs, patterns=None, mode: int = MODE_FP6
     if patterns is not None:
         enc = [_pattern(p) for p in patterns]
         pats = (ctypes.c_char_p * k)(*enc)
-    o = _opts(mode, moduli, flags)
-    _check(load().caps_multiply_batch_device(k, pa, pb, pc, D, pats, ctypes.byref(o),
-                                             _stream_ptr(stream)))
+    cut = list(cutoffs) if cutoffs is not None else [0] * k
+    O = (Options * k)(*[_opts(mode, moduli, flags, c) for c in cut])
+    _check(load().caps_multiply_batch_device(k, pa, pb, pc, D, pats, O, _stream_ptr(stream)))
 
 
 # ------------------------------------------------------------------ layout

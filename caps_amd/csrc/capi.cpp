@@ -528,12 +528,23 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
   if (rc) return rc;
   if (count <= 0 || !dA || !dB || !dC || !descs)
     return fail(CAPS_EINVAL, "caps_multiply_batch_device: bad arguments");
-  if (opts && opts->mode == CAPS_MODE_AUTO)
-    return fail(CAPS_EUNSUPPORTED, "batched calls take an explicit leaf mode");
-  int mode, moduli;
-  rc = resolve_mode(descs[0], opts, mode, moduli);
-  if (rc) return rc;
-  const int cutoff = opts ? opts->cutoff : 0;
+  // opts: NULL (defaults for every item) or one caps_opts per item; the
+  // items share one leaf mode and one modulus count
+  int mode = CAPS_MODE_FP64, moduli = 0;
+  for (int i = 0; i < count; i++) {
+    const caps_opts *oi = opts ? &opts[i] : nullptr;
+    if (oi && oi->mode == CAPS_MODE_AUTO)
+      return fail(CAPS_EUNSUPPORTED, "batched calls take an explicit leaf mode");
+    int mi_, ni_;
+    rc = resolve_mode(descs[i], oi, mi_, ni_);
+    if (rc) return rc;
+    if (i == 0) {
+      mode = mi_;
+      moduli = ni_;
+    } else if (mi_ != mode || ni_ != moduli) {
+      return fail(CAPS_EINVAL, "batch item %d: every item must use the same mode and moduli", i);
+    }
+  }
   std::vector<BatchItem> items((size_t)count);
   std::string key;
   key.reserve((size_t)count * 64);
@@ -541,6 +552,7 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
     rc = check_desc(&descs[i]);
     if (rc) return rc;
     if (!dA[i] || !dB[i] || !dC[i]) return fail(CAPS_EINVAL, "NULL matrix pointer in item %d", i);
+    const int cutoff = opts ? opts[i].cutoff : 0;
     rc = check_cutoff(descs[i], mode, cutoff);
     if (rc) return rc;
     if (mode == CAPS_MODE_CRT_INT8 && descs[i].nrec > 7)
@@ -549,16 +561,18 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
     items[i].B = dB[i];
     items[i].C = dC[i];
     items[i].desc = descs[i];
+    items[i].cutoff = cutoff;
     caps_opts o{mode, moduli, 0, cutoff};
     rc = resolve_pattern(descs[i], patterns ? patterns[i] : nullptr, &o, items[i].pattern, mode,
                          moduli, s.available_doubles / count);
     if (rc) return rc;
-    char buf[160];
-    snprintf(buf, sizeof(buf), "%p %p %p %d %d %d %s;", (const void *)dA[i], (const void *)dB[i],
-             (void *)dC[i], descs[i].lda, descs[i].nrec, descs[i].bs, items[i].pattern.c_str());
+    char buf[176];
+    snprintf(buf, sizeof(buf), "%p %p %p %d %d %d %d %s;", (const void *)dA[i],
+             (const void *)dB[i], (void *)dC[i], descs[i].lda, descs[i].nrec, descs[i].bs, cutoff,
+             items[i].pattern.c_str());
     key += buf;
   }
-  PlanKey pk{nullptr, nullptr, nullptr, descs[0], std::string(), mode, moduli, cutoff, true, key};
+  PlanKey pk{nullptr, nullptr, nullptr, descs[0], std::string(), mode, moduli, 0, true, key};
   Plan *plan = nullptr;
   for (auto &p : s.plans)
     if (p->key == pk) {
@@ -569,7 +583,7 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
     rc = make_plan_locked(
         s, pk,
         [&](Plan &pl, void *work) {
-          return build_batch_plan(pl, items, mode, moduli, cutoff, work);
+          return build_batch_plan(pl, items, mode, moduli, work);
         },
         plan);
     if (rc) return rc;

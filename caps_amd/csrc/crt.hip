@@ -12,7 +12,9 @@ namespace {
 constexpr int T = 256;
 
 // ------------------------------------------------------------------ scale
-// partial (e, s) of row i of A over columns [lj0, lj1) of leaf column block bj
+// partial (e, s) of row i of A over columns [lj0, lj1) of leaf column block bj.
+// PLAIN: column-major leaves (bs == 1 or bs == m), 8 columns per step.
+template <bool PLAIN>
 __global__ __launch_bounds__(T) void crt_rowpart_kernel(const CrtMats *__restrict__ M) {
   const int n = M->n, m = M->m, nrec = M->nrec, bs = M->bs, ch = M->chunks;
   const int i = blockIdx.x * T + threadIdx.x;
@@ -23,12 +25,29 @@ __global__ __launch_bounds__(T) void crt_rowpart_kernel(const CrtMats *__restric
   const int lj0 = k * per, lj1 = min(m, lj0 + per);
   int e = NOSCALE;
   double s = 0.0;
-  for (int lj = lj0; lj < lj1; lj++)
-    sq_acc(M->A[caps_elem_offset(n, nrec, bs, i, bj * m + lj)], e, s);
+  if (PLAIN) {
+    const double *row = M->A + caps_elem_offset(n, nrec, bs, i, bj * m);  // (i, bj m)
+    int lj = lj0;
+    for (; lj + 8 <= lj1; lj += 8) {
+      double v[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) v[t] = row[(long long)(lj + t) * m];
+      int e8;
+      double s8;
+      sq_block8(v, e8, s8);
+      sq_merge(e, s, e8, s8);
+    }
+    for (; lj < lj1; lj++) sq_acc(row[(long long)lj * m], e, s);
+  } else {
+    for (int lj = lj0; lj < lj1; lj++)
+      sq_acc(M->A[caps_elem_offset(n, nrec, bs, i, bj * m + lj)], e, s);
+  }
   reinterpret_cast<double2 *>(M->pa)[(long long)c * n + i] = make_double2(s, (double)e);
 }
 
-// partial of column j of B over leaf row block bi: one wave, fixed butterfly
+// partial of column j of B over leaf row block bi: one wave, fixed butterfly.
+// PLAIN: column-major leaves, each lane takes 8 rows (stride 64) per step.
+template <bool PLAIN>
 __global__ __launch_bounds__(T) void crt_colpart_kernel(const CrtMats *__restrict__ M) {
   const int n = M->n, m = M->m, nrec = M->nrec, bs = M->bs;
   const long long w = (long long)blockIdx.x * (T / 64) + (threadIdx.x >> 6);
@@ -38,8 +57,23 @@ __global__ __launch_bounds__(T) void crt_colpart_kernel(const CrtMats *__restric
   const int j = (int)(w % n), bi = (int)(w / n);
   int e = NOSCALE;
   double s = 0.0;
-  for (int li = lane; li < m; li += 64)
-    sq_acc(M->B[caps_elem_offset(n, nrec, bs, bi * m + li, j)], e, s);
+  if (PLAIN) {
+    const double *col = M->B + caps_elem_offset(n, nrec, bs, bi * m, j);  // (bi m, j)
+    int li = lane;
+    for (; li + 7 * 64 < m; li += 8 * 64) {
+      double v[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) v[t] = col[li + 64 * t];
+      int e8;
+      double s8;
+      sq_block8(v, e8, s8);
+      sq_merge(e, s, e8, s8);
+    }
+    for (; li < m; li += 64) sq_acc(col[li], e, s);
+  } else {
+    for (int li = lane; li < m; li += 64)
+      sq_acc(M->B[caps_elem_offset(n, nrec, bs, bi * m + li, j)], e, s);
+  }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     int e2 = __shfl_down(e, off, 64);
@@ -325,11 +359,18 @@ hipError_t launch_crt_step(int step, const CrtMats *M, const CrtMats &h, const O
   if (leaves > 65535) return hipErrorInvalidValue;
   switch (step) {
     case 0: {
+      const bool plain = (h.bs == 1 || h.bs == h.m);
       dim3 g1((h.n + T - 1) / T, nblk * h.chunks);
-      hipLaunchKernelGGL(crt_rowpart_kernel, g1, dim3(T), 0, s, M);
+      if (plain)
+        hipLaunchKernelGGL((crt_rowpart_kernel<true>), g1, dim3(T), 0, s, M);
+      else
+        hipLaunchKernelGGL((crt_rowpart_kernel<false>), g1, dim3(T), 0, s, M);
       const long long waves = (long long)h.n * nblk;
       dim3 g2((unsigned)((waves + 3) / 4));
-      hipLaunchKernelGGL(crt_colpart_kernel, g2, dim3(T), 0, s, M);
+      if (plain)
+        hipLaunchKernelGGL((crt_colpart_kernel<true>), g2, dim3(T), 0, s, M);
+      else
+        hipLaunchKernelGGL((crt_colpart_kernel<false>), g2, dim3(T), 0, s, M);
       dim3 g3((h.n + T - 1) / T);
       hipLaunchKernelGGL(crt_finalize_kernel, g3, dim3(T), 0, s, M, kc);
       return hipGetLastError();

@@ -578,7 +578,7 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
   b.es = b.residue ? 1 : 8;
   b.planes = b.residue ? moduli : 1;
   if (b.residue) {
-    BatchItem it{A, B, C, d, pattern};
+    BatchItem it{A, B, C, d, pattern, cutoff};
     b.crt_group({&it});
   } else {
     std::vector<Prob> top{{{reinterpret_cast<char *>(const_cast<double *>(A)), 0},
@@ -591,19 +591,20 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
 }
 
 int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, int moduli,
-                     int cutoff, void *work_base) {
+                     void *work_base) {
   plan.steps.clear();
   plan.host_tables.clear();
   plan.additions = plan.multiplications = plan.leaves = 0;
   plan.exec_pattern.clear();
   plan.oz_moduli = 0;
   plan.crt_host.clear();
-  // group by (descriptor, pattern), in first-appearance order
+  // group by (descriptor, pattern, cutoff), in first-appearance order
   std::vector<std::vector<const BatchItem *>> groups;
   for (const BatchItem &it : items) {
     bool placed = false;
     for (auto &g : groups)
-      if (memcmp(&g[0]->desc, &it.desc, sizeof(caps_desc)) == 0 && g[0]->pattern == it.pattern) {
+      if (memcmp(&g[0]->desc, &it.desc, sizeof(caps_desc)) == 0 && g[0]->pattern == it.pattern &&
+          g[0]->cutoff == it.cutoff) {
         g.push_back(&it);
         placed = true;
         break;
@@ -616,7 +617,7 @@ int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, 
     b.bs = g[0]->desc.bs;
     b.mode = mode;
     b.moduli = moduli;
-    b.cutoff = cutoff;
+    b.cutoff = g[0]->cutoff;
     b.pattern = g[0]->pattern;
     b.residue = (mode == CAPS_MODE_CRT_INT8);
     b.es = b.residue ? 1 : 8;

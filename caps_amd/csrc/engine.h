@@ -89,7 +89,8 @@ struct BatchItem {
   const double *A, *B;
   double *C;
   caps_desc desc;
-  std::string pattern;  // resolved, one char per level
+  std::string pattern;  // resolved, one char per recursed level
+  int cutoff = 0;       // layout levels multiplied classically
 };
 
 // Build a plan: pattern must already be resolved to one char per level.
@@ -102,7 +103,7 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C,
 // items with the same (descriptor, pattern) share batched steps; groups run
 // one after another and reuse the same workspace.
 int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, int moduli,
-                     int cutoff, void *work_base);
+                     void *work_base);
 
 // Enqueue every step of the plan on `s` (no graph).
 hipError_t run_steps(const Plan &plan, hipStream_t s);

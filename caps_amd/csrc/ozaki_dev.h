@@ -21,6 +21,22 @@ __device__ __forceinline__ void sq_acc(double a, int &e, double &s) {
   s = fma(t, t, s);
 }
 
+// (exponent, sum of squares) of 8 values: scaled by their largest exponent,
+// so the 8 squares add without rescaling steps
+__device__ __forceinline__ void sq_block8(const double (&v)[8], int &e, double &s) {
+  int E = NOSCALE;
+#pragma unroll
+  for (int k = 0; k < 8; k++) E = v[k] != 0.0 ? max(E, ilogb(v[k])) : E;
+  e = E;
+  s = 0.0;
+  if (E == NOSCALE) return;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const double t = ldexp(v[k], -E);
+    s = fma(t, t, s);
+  }
+}
+
 __device__ __forceinline__ void sq_merge(int &e, double &s, int e2, double s2) {
   if (e2 == NOSCALE) return;
   if (e == NOSCALE) {

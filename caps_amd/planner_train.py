@@ -1,0 +1,178 @@
+"""Train the planner's cost / error model on measured runs and evaluate it on
+held-out shapes against the exhaustive autotuner.
+
+    python -m caps_amd.planner_train --train RUNS_TRAIN.npz ... --test RUNS_TEST.npz ... \\
+        --epochs 300 --model caps_amd/planner_model.npz
+
+Runs come from caps_amd.planner_data.  The model is the library's 3-layer
+perceptron (8 -> 64 -> 64 -> 2, ReLU, FP32), trained by the library's own HIP
+forward / backward / Adam kernels (caps_mlp_train) on normalised features
+(log2 n, exponent spread, #b, #d, cutoff, moduli, residue mode, leaf-split
+mode) and targets (log2 ms, log10 normwise error).
+
+Evaluation per held-out shape, with the error target 1e-14:
+  exhaustive  the fastest measured candidate whose measured error meets the
+              target (what caps_autotune returns);
+  model       the candidate with the smallest predicted time among those
+              whose predicted log10 error is at least a quarter-decade below
+              the target (the library's rule in CAPS_MODE_AUTO); shapes
+              where no candidate qualifies fall back to the exhaustive search;
+  default     the choice without a model or tuning table: residue-domain
+              mode, 14 moduli, all levels batched, no cutoff;
+  reference   the reference's one-process rule: plain FP64, all levels 'd'.
+Regret = t(choice) / t(exhaustive) - 1; a choice whose measured error
+misses the target is a violation.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+
+import numpy as np
+
+import caps_amd
+
+TARGET = 1e-14
+MARGIN = 0.25  # decades, as in the library's planner
+
+
+def load_runs(paths) -> dict:
+    parts = [dict(np.load(p)) for p in paths]
+    out = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    # shape ids are unique within a split; parts of one split never overlap
+    return out
+
+
+def targets(runs):
+    T = np.stack([np.log2(np.maximum(runs["ms"], 1e-6)),
+                  np.log10(np.maximum(runs["err"], 1e-17))], axis=1).astype(np.float32)
+    return T
+
+
+def normaliser(X, T):
+    xm, xs = X.mean(0), X.std(0)
+    xs = np.where(xs > 1e-6, xs, 1.0)
+    tm, ts = T.mean(0), T.std(0)
+    ts = np.where(ts > 1e-6, ts, 1.0)
+    return np.concatenate([xm, xs, tm, ts]).astype(np.float32)
+
+
+def fit(train, epochs=300, seed=0, lr=2e-3, batch=256):
+    X = np.asarray(train["features"], dtype=np.float32)
+    T = targets(train)
+    norm = normaliser(X, T)
+    Xn = (X - norm[:8]) / norm[8:16]
+    Tn = (T - norm[16:18]) / norm[18:20]
+    M = np.ones_like(Tn)
+    model = caps_amd.MLP(seed=seed)
+    # three stages of decreasing step size
+    losses = []
+    for frac, scale in ((0.5, 1.0), (0.3, 0.3), (0.2, 0.1)):
+        losses.append(model.train(Xn, Tn, M, epochs=max(1, int(epochs * frac)), batch=batch,
+                                  lr=lr * scale, seed=seed + len(losses)))
+    return model, norm, losses
+
+
+def predict(model, norm, X):
+    Y = model.forward((np.asarray(X, np.float32) - norm[:8]) / norm[8:16])
+    return Y * norm[18:20] + norm[16:18]  # (log2 ms, log10 err)
+
+
+def _default_index(rows_idx, runs):
+    for j in rows_idx:
+        p = str(runs["pattern"][j])
+        if (runs["mode"][j] == caps_amd.MODE_CRT_INT8 and runs["moduli"][j] == 14 and
+                runs["cutoff"][j] == 0 and set(p) <= {"b"}):
+            return j
+    return None
+
+
+def _reference_index(rows_idx, runs):
+    for j in rows_idx:
+        p = str(runs["pattern"][j])
+        if runs["mode"][j] == caps_amd.MODE_FP64 and runs["cutoff"][j] == 0 and set(p) <= {"d"}:
+            return j
+    return None
+
+
+def evaluate(model, norm, test) -> dict:
+    P = predict(model, norm, test["features"])
+    T = targets(test)
+    res = {"rmse_log2_ms": float(np.sqrt(np.mean((P[:, 0] - T[:, 0]) ** 2))),
+           "rmse_log10_err": float(np.sqrt(np.mean((P[:, 1] - T[:, 1]) ** 2)))}
+    shapes = np.unique(test["shape"])
+    reg = {"model": [], "default": [], "reference": []}
+    viol = {"model": 0, "default": 0, "reference": 0}
+    fallbacks = 0
+    no_feasible = 0
+    for s in shapes:
+        idx = np.nonzero(test["shape"] == s)[0]
+        ms, err = test["ms"][idx], test["err"][idx]
+        feas = err <= TARGET
+        if not feas.any():
+            no_feasible += 1
+            continue
+        best = ms[feas].min()
+        ok = P[idx, 1] <= np.log10(TARGET) - MARGIN
+        if ok.any():
+            k = idx[ok][np.argmin(P[idx[ok], 0])]
+        else:
+            fallbacks += 1
+            k = idx[feas][np.argmin(ms[feas])]
+        for name, j in (("model", k), ("default", _default_index(idx, test)),
+                        ("reference", _reference_index(idx, test))):
+            if j is None:
+                continue
+            reg[name].append(test["ms"][j] / best - 1.0)
+            viol[name] += int(test["err"][j] > TARGET)
+    for name, r in reg.items():
+        r = np.array(r) if r else np.array([np.nan])
+        res[name] = {"shapes": int(np.isfinite(r).sum()), "mean_regret": float(np.nanmean(r)),
+                     "median_regret": float(np.nanmedian(r)),
+                     "p90_regret": float(np.nanpercentile(r, 90)), "violations": viol[name]}
+    res["shapes"] = int(len(shapes))
+    res["model_fallbacks_to_exhaustive"] = fallbacks
+    res["shapes_without_feasible_candidate"] = no_feasible
+    return res
+
+
+def save_model(path, model, norm):
+    np.savez(path, params=model.params(), norm=norm)
+
+
+def load_model(path, install=True):
+    """Load a saved model; with install=True it becomes the planner of
+    CAPS_MODE_AUTO calls (after any tuning-table entry)."""
+    with np.load(path) as z:
+        params, norm = z["params"], z["norm"]
+    model = caps_amd.MLP()
+    model.set_params(params)
+    if install:
+        model.use_as_planner(norm)
+    return model, norm
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", nargs="+", required=True)
+    ap.add_argument("--test", nargs="+", required=True)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--model", default=None, help="save the trained model here (.npz)")
+    a = ap.parse_args(argv)
+    train, test = load_runs(a.train), load_runs(a.test)
+    model, norm, losses = fit(train, a.epochs, a.seed)
+    rep = evaluate(model, norm, test)
+    rep["train_rows"] = int(len(train["ms"]))
+    rep["test_rows"] = int(len(test["ms"]))
+    rep["train_shapes"] = int(len(np.unique(train["shape"])))
+    rep["final_losses"] = losses
+    if a.model:
+        save_model(a.model, model, norm)
+    print(json.dumps(rep, indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -130,10 +130,12 @@ int caps_multiply_device(const double *dA, const double *dB, double *dC,
                          const caps_opts *opts, void *stream);
 
 /* `count` independent multiplies C_i = A_i * B_i of possibly different
- * sizes and depths (device pointers, CAPS layout each), one leaf mode for
- * all (opts; CAPS_MODE_AUTO is not accepted), patterns[i] or NULL per item.
- * Items with the same descriptor and pattern share batched launches; all
- * launches of the call go into one captured hipGraph replayed as one launch. */
+ * sizes and depths (device pointers, CAPS layout each), patterns[i] or NULL
+ * per item.  opts is NULL (defaults) or an array of `count` caps_opts: one
+ * leaf mode (CAPS_MODE_AUTO is not accepted) and one modulus count for all
+ * items, a cutoff per item, flags from opts[0].  Items with the same
+ * descriptor, pattern and cutoff share batched launches; all launches of the
+ * call go into one captured hipGraph replayed as one launch. */
 int caps_multiply_batch_device(int count, const double *const *dA, const double *const *dB,
                                double *const *dC, const caps_desc *descs,
                                const char *const *patterns, const caps_opts *opts,

@@ -5,6 +5,8 @@ for bit; on real inputs it is bit-identical to the same problem run alone
 (the leaf GEMM accumulates in MFMA order, so real-data FP64 results match the
 oracle only to rounding).  Residue-domain mode: the exact-integer model's
 bits on real inputs."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -118,3 +120,48 @@ def test_batch_rejects_auto_and_bad_items():
     bad = caps_amd.MatDescriptor.one_device(64, 2, 3)   # 16 % 3 != 0
     with pytest.raises(caps_amd.CapsError):
         caps_amd.multiply_batch_device([x], [x], [x], [bad], None, caps_amd.MODE_FP64)
+
+
+def test_batch_per_item_cutoff_and_mode_agreement():
+    """Per-item cutoffs: FP64 results equal single calls with the same
+    cutoff; residue-domain results equal the exact model.  Items whose modes
+    differ are rejected."""
+    rng = np.random.default_rng(12)
+    spec = [(512, 2, "bb", 0), (512, 2, "b", 1), (256, 1, "b", 0), (512, 2, "d", 1)]
+    probs = []
+    for n, r, pat, cut in spec:
+        A = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-8, 9, (n, n)))
+        B = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-8, 9, (n, n)))
+        probs.append((A, B, n, r, pat, cut))
+
+    def run_batch(mode, moduli):
+        print(f"[gpu] batch per-item cutoff mode={mode}", flush=True)
+        As = [torch.from_numpy(oracle.to_caps(A, r, 1)).cuda() for A, B, n, r, p, c in probs]
+        Bs = [torch.from_numpy(oracle.to_caps(B, r, 1)).cuda() for A, B, n, r, p, c in probs]
+        Cs = [torch.empty_like(a) for a in As]
+        descs = [caps_amd.MatDescriptor.one_device(n, r, 1) for A, B, n, r, p, c in probs]
+        caps_amd.multiply_batch_device(As, Bs, Cs, descs, [p[4] for p in probs], mode, moduli,
+                                       cutoffs=[p[5] for p in probs])
+        torch.cuda.synchronize()
+        return [oracle.from_caps(c.cpu().numpy(), p[2], p[3], 1) for c, p in zip(Cs, probs)]
+
+    for (A, B, n, r, pat, cut), g in zip(probs, run_batch(caps_amd.MODE_FP64, 0)):
+        d = caps_amd.MatDescriptor.one_device(n, r, 1)
+        dA = torch.from_numpy(oracle.to_caps(A, r, 1)).cuda()
+        dB = torch.from_numpy(oracle.to_caps(B, r, 1)).cuda()
+        dC = torch.empty_like(dA)
+        caps_amd.multiply_device(dA, dB, dC, d, pat, caps_amd.MODE_FP64, cutoff=cut)
+        torch.cuda.synchronize()
+        assert np.array_equal(g, oracle.from_caps(dC.cpu().numpy(), n, r, 1)), (n, pat, cut)
+    for (A, B, n, r, pat, cut), g in zip(probs, run_batch(caps_amd.MODE_CRT_INT8, 14)):
+        assert np.array_equal(g, split.crt_mode(A, B, 14)), (n, pat, cut)
+
+    x = torch.zeros(64 * 64, dtype=torch.float64, device="cuda")
+    d = caps_amd.MatDescriptor.one_device(64, 2, 1)
+    O = (caps_amd.Options * 2)(caps_amd._opts(caps_amd.MODE_FP64, 0, 0),
+                               caps_amd._opts(caps_amd.MODE_CRT_INT8, 14, 0))
+    P = ctypes.c_void_p * 2
+    p = P(x.data_ptr(), x.data_ptr())
+    D = (caps_amd.MatDescriptor * 2)(d, d)
+    assert caps_amd.load().caps_multiply_batch_device(2, p, p, p, D, None, O, None) != 0
+    assert b"same mode" in caps_amd.load().caps_last_error()
