@@ -127,7 +127,7 @@ def test_batch_per_item_cutoff_and_mode_agreement():
     cutoff; residue-domain results equal the exact model.  Items whose modes
     differ are rejected."""
     rng = np.random.default_rng(12)
-    spec = [(512, 2, "bb", 0), (512, 2, "b", 1), (256, 1, "b", 0), (512, 2, "d", 1)]
+    spec = [(512, 2, "bb", 0), (512, 2, "bb", 1), (256, 1, "b", 0), (512, 2, "db", 1)]
     probs = []
     for n, r, pat, cut in spec:
         A = rng.uniform(-1, 1, (n, n)) * np.ldexp(1.0, rng.integers(-8, 9, (n, n)))

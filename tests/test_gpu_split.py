@@ -146,7 +146,7 @@ def test_crt_large_tile_gemm_bit_exact_vs_model():
     d = caps_amd.MatDescriptor.one_device(n, nrec, 1)
     dA = torch.from_numpy(oracle.to_caps(A, nrec, 1)).cuda()
     dB = torch.from_numpy(oracle.to_caps(B, nrec, 1)).cuda()
-    for cutoff, pat in ((0, "bb"), (0, "db"), (1, "b"), (1, "d")):
+    for cutoff, pat in ((0, "bb"), (0, "db"), (1, "bb"), (1, "db")):
         dC = torch.empty_like(dA)
         print(f"[gpu] crt n=1024 cutoff={cutoff} pattern={pat}", flush=True)
         caps_amd.multiply_device(dA, dB, dC, d, pat, mode=CRT, moduli=14, cutoff=cutoff)
