@@ -36,6 +36,7 @@ struct OzakiConsts {
   int nmod;
   int npieces;
   int m[OZ_MAX_MODULI];
+  int c14[OZ_MAX_MODULI];                  // 2^14 mod m_l, symmetric (|c14| <= m_l / 2)
   double inv_m[OZ_MAX_MODULI];
   double w[OZ_MAX_MODULI][OZ_MAX_PIECES];  // CRT weight W_l in base 2^37 digits
   double Mp[OZ_MAX_PIECES];                // M in base 2^37 digits

@@ -336,23 +336,27 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
 // the 128 x 128 kernel above.  Needs m0 % 256 == 0.
 //
 // LDS stage layout per operand: 256 rows x 8 chunks of 16 bytes (k), no
-// padding; chunk c of row r sits in slot c ^ swz(r), swz(r) = (r & 7) ^
-// ((r >> 3) & 1), so the b128 reads of 16 lanes (16 consecutive rows, one
-// chunk) touch 16 distinct 16-byte bank groups.
+// padding; chunk c of row r sits in slot c ^ swz(r), swz(r) = ((r >> 2) & 3)
+// << 1.  Measured on gfx950 (SQ_LDS_BANK_CONFLICT over ds_read_b128 address
+// patterns): a b128 read takes 4 cycles of 16 lanes, the 16 lanes of a cycle
+// lie in one half of the wave, and addresses conflict modulo 512 bytes.  With
+// this swizzle the lanes of every such 16-lane set (rows l16 of a 16-row
+// group, chunks lk and lk' of one half-wave) hit distinct 16-byte slots.
 constexpr int JBM = 256, JBK = 128, JT = 512;
 constexpr int JSTAGE = JBM * JBK;  // 32 KB per operand per stage
 
-__device__ __forceinline__ int jswz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+__device__ __forceinline__ int jswz(int r) { return ((r >> 2) & 3) << 1; }
 
 typedef __attribute__((address_space(3))) void *lds_ptr;
 
 // one stage: wave w loads rows 8 (8 j + w) .. + 7 of A and of B, 8 lanes per
-// 128-byte row; swz(r) = (lane >> 3) ^ (w & 1) for those rows
+// 128-byte row; for those rows swz(r) = (((w & 1) << 1) | (rl >> 2)) << 1
 __device__ __forceinline__ void stage_load(const int8_t *__restrict__ Ab,
                                            const int8_t *__restrict__ Bb, long long ld,
                                            int8_t *As, int8_t *Bs, int wave, int lane) {
   const int rl = lane >> 3;
-  const int c = (lane & 7) ^ rl ^ (wave & 1);  // chunk this lane fetches (same for every j)
+  // chunk this lane fetches (same for every j): slot (lane & 7) holds chunk slot ^ swz(r)
+  const int c = (lane & 7) ^ ((((wave & 1) << 1) | (rl >> 2)) << 1);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const int r = j * 64 + wave * 8 + rl;
@@ -374,6 +378,7 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
   const int l16 = lane & 15, lk = lane >> 4;
   const int G = gridDim.x;
   const int nk = m / JBK;
+  const bool small_k = m <= 16384;  // |sums| <= m 2^14 < 2^28: the cheap exact reduction
   // read offsets (bytes) of this lane's row l16, chunk lk (kh = 0) and lk + 4 (kh = 1)
   const int s16 = jswz(l16);
   const int rd0 = l16 * JBK + ((lk ^ s16) << 4);
@@ -390,7 +395,8 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
     const int8_t *__restrict__ Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major leaves
     int8_t *__restrict__ Cp = tab[leaf].c + l * tab[leaf].sc;
     const int mi = kc->m[l];
-    const float invf = 1.0f / (float)mi;
+    const float mf = (float)mi, invf = 1.0f / mf;
+    const int c14 = kc->c14[l];
     const int row0 = ti * JBM, col0 = tj * JBM;
 
     v4i acc[8][4];
@@ -401,18 +407,23 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
 
     const long long mm0 = (long long)m0 * m0;
     const long long arow = (long long)(row0 % m0) * m0, bcol = (long long)(col0 % m0) * m0;
+    // offsets of the (row0, k) slice of A and the (k, col0) slice of B
+    // (d == 0: plain leaves, no quadtree arithmetic in the k loop)
+    auto a_off = [&](int k) -> long long {
+      return d == 0 ? arow + k : quad_leaf(row0 / m0, k / m0, d) * mm0 + arow + (k % m0);
+    };
+    auto b_off = [&](int k) -> long long {
+      return d == 0 ? bcol + k : quad_leaf(k / m0, col0 / m0, d) * mm0 + bcol + (k % m0);
+    };
     // (the previous tile's k loop ended with a barrier after its last LDS reads)
-    stage_load(Ap + quad_leaf(row0 / m0, 0, d) * mm0 + arow,
-               Bp + quad_leaf(0, col0 / m0, d) * mm0 + bcol, m0, A0, B0, wave, lane);
+    stage_load(Ap + a_off(0), Bp + b_off(0), m0, A0, B0, wave, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     for (int kt = 0; kt < nk; kt++) {
       const bool odd = kt & 1;
       if (kt + 1 < nk) {
         const int k1 = (kt + 1) * JBK;
-        stage_load(Ap + quad_leaf(row0 / m0, k1 / m0, d) * mm0 + arow + (k1 % m0),
-                   Bp + quad_leaf(k1 / m0, col0 / m0, d) * mm0 + bcol + (k1 % m0), m0,
-                   odd ? A0 : A1, odd ? B0 : B1, wave, lane);
+        stage_load(Ap + a_off(k1), Bp + b_off(k1), m0, odd ? A0 : A1, odd ? B0 : B1, wave, lane);
       }
       const int8_t *Ab = (odd ? A1 : A0) + aoff;
       const int8_t *Bb = (odd ? B1 : B0) + boff;
@@ -443,8 +454,14 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
         const int row = (row0 % m0) + wm * 128 + r * 16 + 4 * lk;
         const int col = (col0 % m0) + wn * 64 + c * 16 + l16;
         unsigned v = 0;
+        if (small_k) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
+          for (int q = 0; q < 4; q++)
+            v |= (unsigned)(mod_acc(acc[r][c][q], c14, mf, invf) & 0xff) << (8 * q);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
+        }
         *reinterpret_cast<unsigned *>(Cl + (long long)col * m0 + row) = v;
       }
   }

@@ -75,13 +75,23 @@ __device__ __forceinline__ int sym_residue(double x, int mi, double inv) {
   return ri;
 }
 
-// residue of a small integer (|x| <= 2^20) modulo mi in [-mi/2, mi/2]: x / mi
-// is never a half-integer for odd mi (2x = odd * mi is impossible), and for
-// mi = 256 a tie gives +-128, the same INT8 value after the wrap.  Exact:
-// every step is on integers below 2^24 in FP32.
+// residue of a small integer (|x| < 2^22) modulo mi in [-mi/2, mi/2]: the
+// FP32 quotient estimate x * inv is within |x/mi| 2^-23 < 1 / (2 mi) of x / mi,
+// which is never a half-integer for odd mi (2x = odd * mi is impossible), so
+// rint finds the nearest integer; for mi = 256 (exact inverse) a tie gives
+// +-128, the same INT8 value after the wrap.  Every step is exact: integers
+// below 2^24 in FP32.
 __device__ __forceinline__ int mod_small(int x, float mf, float inv) {
   const float xf = (float)x;
   return (int)fmaf(-rintf(xf * inv), mf, xf);
+}
+
+// residue of an INT32 product sum with |x| < 2^28 (K <= 2^14 products of
+// INT8 residues): x = xh 2^14 + xl with 0 <= xl < 2^14, 2^14 = c14 (mod mi),
+// |xh c14| <= 2^14 * 128, so y = xh c14 + xl < 2^22 goes to mod_small.
+__device__ __forceinline__ int mod_acc(int x, int c14, float mf, float inv) {
+  const int y = __mul24(x >> 14, c14) + (x & 16383);
+  return mod_small(y, mf, inv);
 }
 
 // symmetric residue of an INT32 product sum (|x| < 2^31) modulo mi

@@ -124,6 +124,11 @@ bool ozaki_make_consts(int nmod, OzakiConsts &k) {
   Big M(1);
   for (int l = 0; l < nmod; l++) {
     k.m[l] = mods[l];
+    {
+      int c = (1 << 14) % mods[l];
+      if (c > (mods[l] - 1) / 2) c -= mods[l];
+      k.c14[l] = c;
+    }
     k.inv_m[l] = 1.0 / (double)mods[l];
     M.mul((uint32_t)mods[l]);
   }
