@@ -135,56 +135,50 @@ __global__ __launch_bounds__(T) void bb_post_f64_kernel(const BBPost *__restrict
 }
 
 // ---------------------------------------------------------------- residue pre
-// thread: 4 consecutive positions (4-byte accesses); grid.y: side; grid.z:
-// modulus.  The 7 depth-2 operands of one level-1 operand are finished and
-// stored before the next, so only the 16 input words and 7 output words stay
-// live.  Depth-2 values are sums of at most 16 residues: |x| <= 2048.
+// thread: 2 consecutive positions (2-byte accesses); grid.y: modulus; one
+// instantiation per side.  Depth-2 values are sums of at most 16 residues:
+// |x| <= 2048.  Offsets within a plane are 32-bit (q < 2^31).
+template <int SIDE>
 __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__ P, long long q,
                                                        const OzakiConsts *__restrict__ kc) {
-  const int side = blockIdx.y, l = blockIdx.z;
+  const int l = blockIdx.y;
   const int mi = kc->m[l];
   const float mf = (float)mi, inv = 1.0f / (float)mi;
-  const int8_t *in = reinterpret_cast<const int8_t *>(P->in[side]) + l * P->sin[side];
-  char *obase = reinterpret_cast<char *>(P->out[side]) + l * P->sout[side];
-  const long long n4 = q >> 2;
-  for (long long u = (long long)blockIdx.x * T + threadIdx.x; u < n4;
-       u += (long long)gridDim.x * T) {
-    int k = 0;  // index among the written operands (compile-time after unrolling)
+  const int8_t *in = reinterpret_cast<const int8_t *>(P->in[SIDE]) + l * P->sin[SIDE];
+  char *obase = reinterpret_cast<char *>(P->out[SIDE]) + l * P->sout[SIDE];
+  const long long step = P->step;
+  const unsigned qq = (unsigned)q, n2 = (unsigned)(q >> 1);
+  for (unsigned u = blockIdx.x * T + threadIdx.x; u < n2; u += gridDim.x * T) {
     unsigned w[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-      for (int b = 0; b < 4; b++) w[a][b] = reinterpret_cast<const unsigned *>(in + (4 * a + b) * q)[u];
+      for (int b = 0; b < 4; b++)
+        w[a][b] = *reinterpret_cast<const unsigned short *>(in + (4 * a + b) * qq + 2 * u);
+    int l1[2][7][4];  // [position][level-1 operand i][sub-position b]
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        int qv[4], o[7];
+#pragma unroll
+        for (int a = 0; a < 4; a++) qv[a] = __builtin_amdgcn_sbfe(w[a][b], 8 * h, 8);
+        level_ops(SIDE, qv, o);
+#pragma unroll
+        for (int i = 0; i < 7; i++) l1[h][i][b] = o[i];
+      }
+    int k = 0;  // index among the written operands (compile-time after unrolling)
 #pragma unroll
     for (int i = 0; i < 7; i++) {
-      // recompute from the 16 input words for each i rather than keep the
-      // 112 level-1 values of all i live (an empty asm hides the reuse)
-#pragma unroll
-      for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) asm volatile("" : "+v"(w[a][b]));
-      unsigned out[7] = {0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-      for (int byte = 0; byte < 4; byte++) {
-        int l1[4];  // level-1 operand i at the four sub-positions
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-          int qv[4], o[7];
-#pragma unroll
-          for (int a = 0; a < 4; a++) qv[a] = __builtin_amdgcn_sbfe(w[a][b], 8 * byte, 8);
-          level_ops(side, qv, o);
-          l1[b] = o[i];
-        }
-        int o[7];
-        level_ops(side, l1, o);
-#pragma unroll
-        for (int j = 0; j < 7; j++)
-          out[j] |= (unsigned)(mod_small(o[j], mf, inv) & 0xff) << (8 * byte);
-      }
+      int o0[7], o1[7];
+      level_ops(SIDE, l1[0][i], o0);
+      level_ops(SIDE, l1[1][i], o1);
 #pragma unroll
       for (int j = 0; j < 7; j++) {
-        if (bb_plain(side, i) && bb_plain(side, j)) continue;
-        reinterpret_cast<unsigned *>(obase + k * P->step)[u] = out[j];
+        if (bb_plain(SIDE, i) && bb_plain(SIDE, j)) continue;
+        const unsigned v = (unsigned)(mod_small(o0[j], mf, inv) & 0xff) |
+                           ((unsigned)(mod_small(o1[j], mf, inv) & 0xff) << 8);
+        *reinterpret_cast<unsigned short *>(obase + k * step + 2 * u) = (unsigned short)v;
         k++;
       }
     }
@@ -290,9 +284,10 @@ hipError_t launch_bb_pre(const BBPre *P, long long q, bool residue, int nmod,
     dim3 g(blocks_for(q / 2, 2), 2);
     hipLaunchKernelGGL(bb_pre_f64_kernel, g, dim3(T), 0, s, P, q);
   } else {
-    if (q % 4) return hipErrorInvalidValue;
-    dim3 g(blocks_for(q / 4, 2 * nmod), 2, nmod);
-    hipLaunchKernelGGL(bb_pre_res_kernel, g, dim3(T), 0, s, P, q, kc);
+    if (q % 4 || q >= (1LL << 31)) return hipErrorInvalidValue;
+    dim3 g(blocks_for(q / 2, 2 * nmod), nmod);
+    hipLaunchKernelGGL(bb_pre_res_kernel<0>, g, dim3(T), 0, s, P, q, kc);
+    hipLaunchKernelGGL(bb_pre_res_kernel<1>, g, dim3(T), 0, s, P, q, kc);
   }
   return hipGetLastError();
 }
