@@ -124,13 +124,13 @@ def batch_sizes(count=512, seed=2024):
 
 def tune_one_size(torch, caps_amd, n, mode, moduli, reps=2):
     """Fastest (nrec, pattern, cutoff) for one size: layout depths with
-    leaves >= 32, every candidate the library enumerates for the mode at the
+    leaves >= 64, every candidate the library enumerates for the mode at the
     given modulus count, timed on the device (caps_measure)."""
     mask = {caps_amd.MODE_FP64: caps_amd.MASK_FP64,
             caps_amd.MODE_CRT_INT8: caps_amd.MASK_CRT_INT8}[mode]
     best = None
     for r in range(0, 7):
-        if n % (1 << r) or (n >> r) < 32:
+        if n % (1 << r) or (n >> r) < 64:
             continue
         desc = caps_amd.MatDescriptor.one_device(n, r, 1)
         A = torch.rand(n * n, device="cuda", dtype=torch.float64) * 2 - 1
@@ -157,7 +157,11 @@ def batch_workload(args, torch, caps_amd):
     for mode, name in ((caps_amd.MODE_FP64, "fp64"), (caps_amd.MODE_CRT_INT8, "split")):
         log(f"[bench] batch: tuning {len(distinct)} sizes ({name})")
         t0 = time.perf_counter()
-        choice = {n: tune_one_size(torch, caps_amd, n, mode, K) for n in distinct}
+        choice = {}
+        for n in distinct:
+            choice[n] = tune_one_size(torch, caps_amd, n, mode, K)
+            log(f"[bench] batch: n={n} -> nrec={choice[n][0]} pattern={choice[n][1]} "
+                f"cutoff={choice[n][2]} {choice[n][3]:.3f} ms")
         tune_s = time.perf_counter() - t0
         caps_amd.release()
         g = torch.Generator(device="cuda")

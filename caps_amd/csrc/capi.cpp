@@ -73,6 +73,9 @@ State &st() {
 }
 
 constexpr size_t MAX_PLANS = 24;
+// plans with more launches than this (deep 'd' recursions: 7^d leaf
+// products one after another) are launched directly instead of captured
+constexpr size_t MAX_GRAPH_STEPS = 16384;
 
 int ensure_device_locked(State &s) {
   int dev = 0;
@@ -316,7 +319,8 @@ int multiply_device_locked(State &s, const double *dA, const double *dB, double 
   rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, cutoff, fuse, plan);
   if (rc) return rc;
   const bool timed = opts && (opts->flags & CAPS_FLAG_TIMERS);
-  const bool use_graph = !timed && !(opts && (opts->flags & CAPS_FLAG_NO_GRAPH));
+  const bool use_graph = !timed && !(opts && (opts->flags & CAPS_FLAG_NO_GRAPH)) &&
+                         plan->steps.size() <= MAX_GRAPH_STEPS;
   if (timed) {
     hipError_t e = run_steps_timed(*plan, stream, s.timers_ms, s.timed);
     if (e != hipSuccess) return fail(CAPS_EHIP, "timed launch: %s", hipGetErrorString(e));
@@ -588,7 +592,8 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
         plan);
     if (rc) return rc;
   }
-  const bool use_graph = !(opts && (opts->flags & (CAPS_FLAG_NO_GRAPH | CAPS_FLAG_TIMERS)));
+  const bool use_graph = !(opts && (opts->flags & (CAPS_FLAG_NO_GRAPH | CAPS_FLAG_TIMERS))) &&
+                         plan->steps.size() <= MAX_GRAPH_STEPS;
   if (opts && (opts->flags & CAPS_FLAG_TIMERS)) {
     hipError_t e = run_steps_timed(*plan, (hipStream_t)stream, s.timers_ms, s.timed);
     if (e != hipSuccess) return fail(CAPS_EHIP, "timed launch: %s", hipGetErrorString(e));

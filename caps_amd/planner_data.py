@@ -8,7 +8,7 @@ product) are recorded.
         --out gpurun_out/planner_train_0.npz
 
 Shapes: n = 256 round(exp(U[ln 512, ln 16384]) / 256); layout depth r = the
-largest with leaves n / 2^r >= 256 and n divisible by 2^r; entries u 2^e with
+largest with leaves n / 2^r >= 512 and n divisible by 2^r (at least 0); entries u 2^e with
 u uniform [-1, 1), e uniform integer in [-spread, spread], spread uniform in
 {0, ..., 20}.  Training and held-out shapes come from different seeds.
 Candidates: plain FP64 and the residue-domain mode (moduli 12..16), every
@@ -34,7 +34,7 @@ def shape(split: str, i: int):
     n = int(256 * np.round(np.exp(rng.uniform(np.log(512), np.log(16384))) / 256))
     n = min(max(n, 512), 16384)
     r = 0
-    while n % (2 << r) == 0 and (n >> (r + 1)) >= 256:
+    while n % (2 << r) == 0 and (n >> (r + 1)) >= 512:
         r += 1
     spread = int(rng.integers(0, 21))
     return n, r, spread

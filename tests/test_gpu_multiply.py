@@ -315,3 +315,17 @@ def test_fused_levels_bit_identical_to_unfused(n, nrec, pattern):
     assert np.array_equal(fused, plain)
     if n <= 64:
         assert np.array_equal(fused, oracle.multiply(a, b, n, nrec, 1, pattern, 4))
+
+
+def test_deep_dfs_plan_runs_without_graph():
+    """A pattern with 7^5 sequential leaf products (about 33k launches) is
+    above the graph-capture limit and runs as direct launches: same bits as
+    the oracle on integers."""
+    n, r, pat = 1024, 5, "ddddd"
+    rng = np.random.default_rng(55)
+    A = rng.integers(-8, 9, (n, n)).astype(np.float64)
+    B = rng.integers(-8, 9, (n, n)).astype(np.float64)
+    a, b = oracle.to_caps(A, r, 1), oracle.to_caps(B, r, 1)
+    got = run(a, b, n, r, 1, pat)
+    want = oracle.multiply(a, b, n, r, 1, pat, 8)
+    assert np.array_equal(got, want)
