@@ -64,6 +64,22 @@ def time_steps(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def profiled_traffic(kernel, path=os.path.join(ROOT, "profiles", "crt_int8_n8192_bb.md")):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE runs, calibrated by a device copy), or None."""
+    try:
+        in_traffic = False
+        for line in open(path):
+            if line.startswith("## "):
+                in_traffic = line.startswith("## HBM traffic")
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if in_traffic and len(cells) >= 4 and cells[0] == f"`{kernel}`":
+                return round((float(cells[2]) + float(cells[3])) * 1048576)
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def leaf_kernel_ms(caps_amd, torch, dA, dB, dC, desc, pattern, mode, moduli, reps=3):
     """Average duration of the leaf GEMM launch, HIP events on its stream."""
     caps_amd.reset_counters()
@@ -280,6 +296,12 @@ def main():
         caps_amd.multiply_device(dA, dB, dC_leaf, desc, pat, mode=caps_amd.MODE_OZAKI_INT8,
                                  moduli=K)
 
+    dC_bf16 = torch.empty_like(dA)
+
+    def run_bf16():
+        caps_amd.multiply_device(dA, dB, dC_bf16, desc, pat, mode=caps_amd.MODE_CRT_BF16,
+                                 moduli=K)
+
     log("[bench] split precision (residue-domain Strassen, INT8 MFMA, CRT)")
     t_split = time_steps(torch, run_split, args.steps, args.warmup)
     log(f"[bench] split {t_split * 1e3:.3f} ms/step")
@@ -289,6 +311,9 @@ def main():
     log("[bench] leaf-only split (FP64 Winograd additions)")
     t_leaf = time_steps(torch, run_leaf, max(2, args.steps // 2), 1)
     log(f"[bench] leaf split {t_leaf * 1e3:.3f} ms/step")
+    log("[bench] residue-domain Strassen with BF16 MFMA leaf products")
+    t_bf16 = time_steps(torch, run_bf16, max(2, args.steps // 2), 1)
+    log(f"[bench] bf16 variant {t_bf16 * 1e3:.3f} ms/step")
 
     # roofline of the dominant kernel: the leaf GEMM launch (all 49 x K INT8 GEMMs)
     per_launch, per_call = leaf_kernel_ms(caps_amd, torch, dA, dB, dC_split, desc, pat,
@@ -323,16 +348,19 @@ def main():
                    "split_mode": "residue-domain Strassen (CAPS_MODE_CRT_INT8)",
                    "plain_fp64_gflops": round(eff / t_plain / 1e9, 2),
                    "leaf_split_gflops": round(eff / t_leaf / 1e9, 2),
+                   "bf16_variant_gflops": round(eff / t_bf16 / 1e9, 2),
+                   "bf16_variant_ms_per_step": round(t_bf16 * 1e3, 4),
                    "plain_fp64_ms_per_step": round(t_plain * 1e3, 4),
                    "split_over_plain": round(t_plain / t_split, 3),
                    "phases_ms_split": {k: round(v, 4) for k, v in per_call.items() if v},
                    "phases_ms_plain": {k: round(v, 4) for k, v in plain_call.items() if v},
                    "plain_leaf_gemm_tflops": round(f64_ach, 2),
                    "plain_leaf_gemm_frac_fp64_peak": round(f64_ach / PEAK_FP64_TFLOPS, 4)},
-        "roofline": {"bound": "mfma", "kernel": "oz_igemm_kernel (INT8 MFMA, all leaves x moduli)",
+        "roofline": {"bound": "mfma",
+                     "kernel": "oz_igemm_big_kernel (INT8 MFMA, all leaves x moduli)",
                      "achieved": round(achieved, 2), "peak": PEAK_INT8_TOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_INT8_TOPS, 4),
-                     "traffic": None,
+                     "traffic": profiled_traffic("oz_igemm_big_kernel"),
                      "note": "INT8 multiply-adds counted as 2 ops; per launch "
                              f"{int_ops:.4e} ops = moduli x 2 x 49 x 2048^3, "
                              f"{gemm_ms:.3f} ms"},
@@ -350,6 +378,7 @@ def main():
                                   oracle.from_caps(dC_plain.cpu().numpy(), n, r, 1), rows)
         err_leaf = sampled_error(oracle, A_cm, B_cm,
                                  oracle.from_caps(dC_leaf.cpu().numpy(), n, r, 1), rows)
+        bf16_same = bool(torch.equal(dC_bf16, dC_split))
         log(f"[bench] error split {err_split:.3e} plain {err_plain:.3e} leaf {err_leaf:.3e}")
         del A_cm, B_cm
         log("[bench] integer pair through the split path")
@@ -365,6 +394,7 @@ def main():
         result["config"]["error_split"] = err_split
         result["config"]["error_plain"] = err_plain
         result["config"]["error_leaf_split"] = err_leaf
+        result["config"]["bf16_variant_bitwise_equal_to_int8"] = bf16_same
         result["config"]["error_metric"] = ("max|C-C*|/max|C*| over 16 sampled full rows, "
                                             "C* = Dot2 compensated classical product")
         result["config"]["integer_pair_bit_exact"] = exact

@@ -33,6 +33,7 @@ SEVEN = 7  # library.h:6
 MODE_FP64 = 0
 MODE_OZAKI_INT8 = 1
 MODE_CRT_INT8 = 2
+MODE_CRT_BF16 = 3
 MODE_AUTO = 99
 FLAG_NO_GRAPH = 1
 FLAG_REFERENCE_AUTO = 2
@@ -94,6 +95,7 @@ class Candidate(ctypes.Structure):
 MASK_FP64 = 1 << MODE_FP64
 MASK_OZAKI_INT8 = 1 << MODE_OZAKI_INT8
 MASK_CRT_INT8 = 1 << MODE_CRT_INT8
+MASK_CRT_BF16 = 1 << MODE_CRT_BF16
 
 
 # every symbol include/caps_amd.h declares, with (restype, argtypes)

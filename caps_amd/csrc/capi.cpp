@@ -156,11 +156,12 @@ int resolve_mode(const caps_desc &d, const caps_opts *o, int &mode, int &moduli)
   mode = o ? o->mode : CAPS_MODE_FP64;
   moduli = (o && o->moduli) ? o->moduli : OZ_DEFAULT_MODULI;
   if (mode == CAPS_MODE_AUTO) mode = choose_mode(d);
-  if (mode != CAPS_MODE_FP64 && mode != CAPS_MODE_OZAKI_INT8 && mode != CAPS_MODE_CRT_INT8)
+  if (mode != CAPS_MODE_FP64 && mode != CAPS_MODE_OZAKI_INT8 && mode != CAPS_MODE_CRT_INT8 &&
+      mode != CAPS_MODE_CRT_BF16)
     return fail(CAPS_EINVAL, "unknown mode %d", mode);
   if (mode != CAPS_MODE_FP64 && (moduli < 2 || moduli > OZ_MAX_MODULI))
     return fail(CAPS_EINVAL, "moduli=%d outside 2..%d", moduli, OZ_MAX_MODULI);
-  if (mode == CAPS_MODE_CRT_INT8 && d.nrec > 7)
+  if (residue_mode(mode) && d.nrec > 7)
     return fail(CAPS_EUNSUPPORTED, "residue-domain mode supports nrec <= 7 (got %d)", d.nrec);
   if (mode == CAPS_MODE_FP64) moduli = 0;
   return CAPS_OK;
@@ -312,7 +313,7 @@ int multiply_device_locked(State &s, const double *dA, const double *dB, double 
   }
   rc = check_cutoff(*desc, mode, cutoff);
   if (rc) return rc;
-  if (mode == CAPS_MODE_CRT_INT8 && desc->nrec > 7)
+  if (residue_mode(mode) && desc->nrec > 7)
     return fail(CAPS_EUNSUPPORTED, "residue-domain mode supports nrec <= 7 (got %d)", desc->nrec);
   Plan *plan = nullptr;
   const bool fuse = !(opts && (opts->flags & CAPS_FLAG_NO_FUSION));
@@ -559,7 +560,7 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
     const int cutoff = opts ? opts[i].cutoff : 0;
     rc = check_cutoff(descs[i], mode, cutoff);
     if (rc) return rc;
-    if (mode == CAPS_MODE_CRT_INT8 && descs[i].nrec > 7)
+    if (residue_mode(mode) && descs[i].nrec > 7)
       return fail(CAPS_EUNSUPPORTED, "residue-domain mode supports nrec <= 7");
     items[i].A = dA[i];
     items[i].B = dB[i];
@@ -695,7 +696,8 @@ int caps_measure(const double *dA, const double *dB, double *dC, const caps_desc
 }
 
 int caps_planner_config(double max_err, int mode_mask, double exponent_spread) {
-  if (!(max_err > 0) || (mode_mask & ~(CAPS_MASK_FP64 | CAPS_MASK_OZAKI_INT8 | CAPS_MASK_CRT_INT8)) ||
+  if (!(max_err > 0) || (mode_mask & ~(CAPS_MASK_FP64 | CAPS_MASK_OZAKI_INT8 | CAPS_MASK_CRT_INT8 |
+                                        CAPS_MASK_CRT_BF16)) ||
       !mode_mask)
     return fail(CAPS_EINVAL, "caps_planner_config: bad error target or mode mask");
   State &s = st();

@@ -170,7 +170,8 @@ struct Builder {
                 reinterpret_cast<const int8_t *>(batch[p].B.p),
                 reinterpret_cast<int8_t *>(batch[p].C.p), batch[p].A.ps, batch[p].B.ps,
                 batch[p].C.ps};
-      push(STEP_OZ_GEMM, moduli, (int)g.size(), m, 0,
+      // bs field: 1 = BF16 MFMA leaf products
+      push(STEP_OZ_GEMM, moduli, (int)g.size(), m, mode == CAPS_MODE_CRT_BF16 ? 1 : 0,
            add_table(g.data(), g.size() * sizeof(RGemm)), 0, cutoff);
       return;
     }
@@ -509,6 +510,9 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
       return launch_leaf_reorder(reinterpret_cast<const ReorderPtrs *>(tb + st.table), st.count,
                                  (int)st.n, st.bs, st.op == 1, s);
     case STEP_OZ_GEMM:
+      if (st.bs == 1)
+        return launch_residue_gemm_bf16(reinterpret_cast<const RGemm *>(tb + st.table), st.count,
+                                        (int)st.n, st.levels, st.op, kc, s);
       return launch_residue_gemm(reinterpret_cast<const RGemm *>(tb + st.table), st.count,
                                  (int)st.n, st.levels, st.op, kc, s);
     case STEP_OZ_SCALE:
@@ -574,7 +578,7 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
   b.cutoff = cutoff;
   b.fuse = fuse;
   b.pattern = pattern;
-  b.residue = (mode == CAPS_MODE_CRT_INT8);
+  b.residue = residue_mode(mode);
   b.es = b.residue ? 1 : 8;
   b.planes = b.residue ? moduli : 1;
   if (b.residue) {
@@ -619,7 +623,7 @@ int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, 
     b.moduli = moduli;
     b.cutoff = g[0]->cutoff;
     b.pattern = g[0]->pattern;
-    b.residue = (mode == CAPS_MODE_CRT_INT8);
+    b.residue = residue_mode(mode);
     b.es = b.residue ? 1 : 8;
     b.planes = b.residue ? moduli : 1;
     b.fuse = true;

@@ -13,6 +13,11 @@
 
 namespace caps {
 
+// the whole recursion on residues (INT8 or BF16 leaf products)
+inline bool residue_mode(int mode) {
+  return mode == CAPS_MODE_CRT_INT8 || mode == CAPS_MODE_CRT_BF16;
+}
+
 // A matrix operand of a batched level: FP64 (plane stride unused) or nmod
 // INT8 residue planes, plane l at p + l * ps bytes.
 struct Mat {

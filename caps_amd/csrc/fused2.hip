@@ -162,7 +162,7 @@ __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__
       for (int b = 0; b < 4; b++) {
         int qv[4], o[7];
 #pragma unroll
-        for (int a = 0; a < 4; a++) qv[a] = __builtin_amdgcn_sbfe(w[a][b], 8 * h, 8);
+        for (int a = 0; a < 4; a++) qv[a] = (int)__builtin_amdgcn_sbfe(w[a][b], 8 * h, 8);
         level_ops(SIDE, qv, o);
 #pragma unroll
         for (int i = 0; i < 7; i++) l1[h][i][b] = o[i];
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
       for (int i = 0; i < 7; i++) {
         int qv[7], c[4];
 #pragma unroll
-        for (int j = 0; j < 7; j++) qv[j] = __builtin_amdgcn_sbfe(w[7 * i + j], 8 * byte, 8);
+        for (int j = 0; j < 7; j++) qv[j] = (int)__builtin_amdgcn_sbfe(w[7 * i + j], 8 * byte, 8);
         Ew<EW_BFS_POST>::apply(qv, c);
 #pragma unroll
         for (int bb = 0; bb < 4; bb++) l1[i][bb] = c[bb];  // |.| <= 512
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
 #pragma unroll
     for (int p = 0; p < NP; p++) S[p] = 0.0;
     for (int l = 0; l < nmod; l++)
-      crt_acc<NP>(S, __builtin_amdgcn_sbfe(R[l][o][pl >> 2], 8 * (pl & 3), 8), l, kc);
+      crt_acc<NP>(S, (int)__builtin_amdgcn_sbfe(R[l][o][pl >> 2], 8 * (pl & 3), 8), l, kc);
     // position u of a depth-2 sub-block: leaf L2 (quadtree of nrec2 levels), column-major leaf
     const long long L2 = u / mm0, e = u - L2 * mm0;
     const int li = (int)(e % m0), lj = (int)(e / m0);

@@ -86,6 +86,9 @@ hipError_t launch_ozaki_step(int step, const OzLeaf *d_tab, int count, int m, in
 // persistent launch.  Operands are m x m quadtrees of `levels` quadrant levels
 // over leaves of size m >> levels (A leaves row-major, B and C column-major);
 // levels > 0 needs leaves that are multiples of 128.
+// the same products on BF16 MFMA with FP32 accumulation (bit-identical results)
+hipError_t launch_residue_gemm_bf16(const RGemm *d_tab, int count, int m, int levels, int nmod,
+                                    const OzakiConsts *d_consts, hipStream_t s);
 hipError_t launch_residue_gemm(const RGemm *d_tab, int count, int m, int levels, int nmod,
                                const OzakiConsts *d_consts, hipStream_t s);
 

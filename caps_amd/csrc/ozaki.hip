@@ -188,9 +188,10 @@ constexpr int ISTR = IBK + 16;  // LDS row stride (bytes): b128 reads of 16 lane
 
 // global -> registers: 128 rows of A (k contiguous) and 128 columns of B
 // (k contiguous), 64 k each, 16 bytes per access.  Ab / Bb point at the
-// slice's first element, ld is the row (A) / column (B) stride.  CHECK:
-// bounds against m with the slice origin; only for plain operands.
-template <bool CHECK>
+// slice's first element, ld is the row (A) / column (B) stride.  CHECK 0:
+// none; 1: every byte against m; 2: rows / columns against m (m % 64 == 0,
+// so K slices are whole).  Checks use the slice origin; plain operands only.
+template <int CHECK>
 __device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ab,
                                          const int8_t *__restrict__ Bb, long long ld, int m,
                                          int row0, int col0, int k0, int tid, v4u (&ga)[2],
@@ -199,9 +200,14 @@ __device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ab,
   for (int it = 0; it < 2; it++) {
     const int idx = tid + T256 * it;
     const int r = idx >> 2, kq = (idx & 3) * 16;
-    if (!CHECK) {
+    if (CHECK == 0) {
       ga[it] = *reinterpret_cast<const v4u *>(Ab + r * ld + kq);
       gb[it] = *reinterpret_cast<const v4u *>(Bb + r * ld + kq);
+    } else if (CHECK == 2) {
+      // the K slice is whole; rows of A / columns of B beyond m read as zero
+      const v4u z = {0u, 0u, 0u, 0u};
+      ga[it] = row0 + r < m ? *reinterpret_cast<const v4u *>(Ab + r * ld + kq) : z;
+      gb[it] = col0 + r < m ? *reinterpret_cast<const v4u *>(Bb + r * ld + kq) : z;
     } else {
       unsigned wa[4], wb[4];
 #pragma unroll
@@ -239,7 +245,7 @@ __device__ __forceinline__ void lstore_i8(int8_t *As, int8_t *Bs, int tid, const
 // Operands are m x m quadtrees of d levels over leaves of size m0 = m >> d in
 // CAPS quadrant order: A leaves row-major, B and C leaves column-major (d = 0:
 // plain matrices).  The fast path needs m0 % 128 == 0.
-template <bool CHECK>
+template <int CHECK>
 __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict__ tab, int m,
                                                         int nmod,
                                                         const OzakiConsts *__restrict__ kc,
@@ -313,7 +319,7 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
         const int col = col0 + wn * 64 + c * 16 + l16;
         int8_t *dst = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0 +
                       (long long)(col % m0) * m0 + (row % m0);
-        if (!CHECK) {
+        if (CHECK == 0) {
           unsigned v = 0;
 #pragma unroll
           for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
@@ -467,6 +473,160 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
   }
 }
 
+// ------------------------------------------------------------------ BF16 GEMM on residues
+// The BF16 variant of the residue GEMM: INT8 residues (|r| <= 128, exact in
+// BF16) are widened when staged into LDS and multiplied on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  FP32 holds integers
+// exactly below 2^24: after a fold x - m rint(x / m) (|result| <= 1.5 m) at
+// most 512 more products of magnitude <= 2^14 are added, so the running sum
+// stays below 2^24 and every partial sum is exact.  The result equals the
+// INT8 kernel's bit for bit.  128 x 128 tiles, 4 waves of 64 x 64, BK = 64.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int BSTR = IBK * 2 + 16;  // LDS row stride (bytes) of BF16 rows
+constexpr int BFOLD = 512;          // K between folds
+
+// 16 INT8 residues -> 16 BF16 values (8 dwords): exact widening through FP32
+__device__ __forceinline__ void widen16(v4u w, unsigned (&o)[8]) {
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      // (the builtin's result type is unsigned: take it as int before widening)
+      const float f0 = (float)(int)__builtin_amdgcn_sbfe(w[q], 16 * h, 8);
+      const float f1 = (float)(int)__builtin_amdgcn_sbfe(w[q], 16 * h + 8, 8);
+      o[2 * q + h] = (__float_as_uint(f0) >> 16) | (__float_as_uint(f1) & 0xffff0000u);
+    }
+}
+
+__device__ __forceinline__ void lstore_bf(int8_t *As, int8_t *Bs, int tid, const v4u (&ga)[2],
+                                          const v4u (&gb)[2]) {
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const int idx = tid + T256 * it;
+    const int r = idx >> 2, kq = (idx & 3) * 16;  // 16 k values = 32 bytes
+    unsigned o[8];
+    widen16(ga[it], o);
+    *reinterpret_cast<v4u *>(&As[r * BSTR + 2 * kq]) = v4u{o[0], o[1], o[2], o[3]};
+    *reinterpret_cast<v4u *>(&As[r * BSTR + 2 * kq + 16]) = v4u{o[4], o[5], o[6], o[7]};
+    widen16(gb[it], o);
+    *reinterpret_cast<v4u *>(&Bs[r * BSTR + 2 * kq]) = v4u{o[0], o[1], o[2], o[3]};
+    *reinterpret_cast<v4u *>(&Bs[r * BSTR + 2 * kq + 16]) = v4u{o[4], o[5], o[6], o[7]};
+  }
+}
+
+__device__ __forceinline__ float fold_f32(float x, float mf, float inv) {
+  return fmaf(-rintf(x * inv), mf, x);
+}
+
+template <int CHECK>
+__global__ __launch_bounds__(T256) void oz_bgemm_kernel(const RGemm *__restrict__ tab, int m,
+                                                        int nmod,
+                                                        const OzakiConsts *__restrict__ kc,
+                                                        TileMap map, int m0, int d) {
+  __shared__ __attribute__((aligned(16))) int8_t As[2][IBM * BSTR];
+  __shared__ __attribute__((aligned(16))) int8_t Bs[2][IBN * BSTR];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l16 = lane & 15, lk = lane >> 4;
+  const int G = gridDim.x;
+  const int nk = (m + IBK - 1) / IBK;
+  const int fold_every = BFOLD / IBK;
+
+  for (int round = 0; round * G < map.total; round++) {
+    const int tile = round * G + xcd_tile(blockIdx.x, G);
+    if (tile >= map.total) break;
+    int prob, ti, tj;
+    tile_coords(map, tile, prob, ti, tj);
+    const int leaf = prob / nmod, l = prob - leaf * nmod;
+    const int8_t *__restrict__ Ap = tab[leaf].a + l * tab[leaf].sa;  // row-major
+    const int8_t *__restrict__ Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major
+    int8_t *__restrict__ Cp = tab[leaf].c + l * tab[leaf].sc;
+    const int mi = kc->m[l];
+    const float mf = (float)mi, invf = 1.0f / mf;
+    const int row0 = ti * IBM, col0 = tj * IBN;
+
+    v4f acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[r][c] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    v4u ga[2], gb[2];
+    const long long mm0 = (long long)m0 * m0;
+    auto a_at = [&](int k0) {
+      return Ap + quad_leaf(row0 / m0, k0 / m0, d) * mm0 + (long long)(row0 % m0) * m0 + (k0 % m0);
+    };
+    auto b_at = [&](int k0) {
+      return Bp + quad_leaf(k0 / m0, col0 / m0, d) * mm0 + (k0 % m0) + (long long)(col0 % m0) * m0;
+    };
+    gload_i8<CHECK>(a_at(0), b_at(0), m0, m, row0, col0, 0, tid, ga, gb);
+    lstore_bf(As[0], Bs[0], tid, ga, gb);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+      const int buf = kt & 1;
+      if (kt + 1 < nk)
+        gload_i8<CHECK>(a_at((kt + 1) * IBK), b_at((kt + 1) * IBK), m0, m, row0, col0,
+                        (kt + 1) * IBK, tid, ga, gb);
+#pragma unroll
+      for (int kh = 0; kh < 2; kh++) {
+        // lane group lk supplies k = 32 kh + 8 lk .. + 7 for both operands
+        v8bf a[4], b[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          a[r] = *reinterpret_cast<const v8bf *>(
+              &As[buf][(wm * 64 + r * 16 + l16) * BSTR + 2 * (32 * kh + 8 * lk)]);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          b[c] = *reinterpret_cast<const v8bf *>(
+              &Bs[buf][(wn * 64 + c * 16 + l16) * BSTR + 2 * (32 * kh + 8 * lk)]);
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b[c], acc[r][c], 0, 0, 0);
+      }
+      if ((kt + 1) % fold_every == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[r][c][q] = fold_f32(acc[r][c][q], mf, invf);
+      }
+      if (kt + 1 < nk) lstore_bf(As[buf ^ 1], Bs[buf ^ 1], tid, ga, gb);
+      __syncthreads();
+    }
+
+    // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v.
+    // |acc| < 2^24 exact; a fold brings it to |x| <= 1.5 m, then the exact
+    // symmetric residue
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int row = row0 + wm * 64 + r * 16 + 4 * lk;
+        const int col = col0 + wn * 64 + c * 16 + l16;
+        int8_t *dst = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0 +
+                      (long long)(col % m0) * m0 + (row % m0);
+        int res[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          res[q] = mod_small((int)fold_f32(acc[r][c][q], mf, invf), mf, invf);
+        if (CHECK == 0) {
+          unsigned v = 0;
+#pragma unroll
+          for (int q = 0; q < 4; q++) v |= (unsigned)(res[q] & 0xff) << (8 * q);
+          *reinterpret_cast<unsigned *>(dst) = v;
+        } else if (col < m) {
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (row + q < m) dst[q] = (int8_t)res[q];
+        }
+      }
+  }
+}
+
 // ------------------------------------------------------------------ reconstruct
 template <int NP, bool VEC>
 __global__ __launch_bounds__(T256) void oz_recon_kernel(const OzLeaf *__restrict__ tab, int m,
@@ -527,6 +687,30 @@ hipError_t launch_recon(const OzLeaf *tab, int count, int m, int nmod, const Oza
 
 }  // namespace
 
+hipError_t launch_residue_gemm_bf16(const RGemm *tab, int count, int m, int levels, int nmod,
+                                    const OzakiConsts *kc, hipStream_t s) {
+  if (count <= 0 || m <= 0) return hipSuccess;
+  if (nmod < 2 || nmod > OZ_MAX_MODULI) return hipErrorInvalidValue;
+  const int m0 = m >> levels;
+  if (levels < 0 || (m0 << levels) != m) return hipErrorInvalidValue;
+  bool ok;
+  TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);
+  if (!ok) return hipErrorInvalidValue;
+  const int G = persistent_grid(2, device_cu_count(), map.total);
+  const bool fast = (m0 % IBM == 0);
+  if (!fast && levels > 0) return hipErrorInvalidValue;
+  if (fast)
+    hipLaunchKernelGGL((oz_bgemm_kernel<0>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
+                       m0, levels);
+  else if (m % IBK == 0)
+    hipLaunchKernelGGL((oz_bgemm_kernel<2>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
+                       m0, levels);
+  else
+    hipLaunchKernelGGL((oz_bgemm_kernel<1>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
+                       m0, levels);
+  return hipGetLastError();
+}
+
 hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, int nmod,
                                const OzakiConsts *kc, hipStream_t s) {
   if (count <= 0 || m <= 0) return hipSuccess;
@@ -546,15 +730,19 @@ hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, i
   TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);
   if (!ok) return hipErrorInvalidValue;
   const int G = persistent_grid(2, device_cu_count(), map.total);
-  // the fast path reads 16-byte rows: leaves of a multiple of 128
+  // the fast path reads 16-byte rows: leaves of a multiple of 128; with m %
+  // 64 == 0 the K slices are whole and only edge rows / columns are checked
   const bool fast = (m0 % IBM == 0);
   if (!fast && levels > 0) return hipErrorInvalidValue;
   if (fast)
-    hipLaunchKernelGGL((oz_igemm_kernel<false>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
-                       map, m0, levels);
+    hipLaunchKernelGGL((oz_igemm_kernel<0>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
+                       m0, levels);
+  else if (m % IBK == 0)
+    hipLaunchKernelGGL((oz_igemm_kernel<2>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
+                       m0, levels);
   else
-    hipLaunchKernelGGL((oz_igemm_kernel<true>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc,
-                       map, m0, levels);
+    hipLaunchKernelGGL((oz_igemm_kernel<1>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
+                       m0, levels);
   return hipGetLastError();
 }
 

@@ -68,6 +68,10 @@ typedef enum caps_mode {
                                products, one CRT reconstruction of C; exact
                                for integer products below 2^53, otherwise
                                only the initial scaling truncates (nrec <= 7) */
+  CAPS_MODE_CRT_BF16 = 3,   /* as CAPS_MODE_CRT_INT8 with the leaf products on
+                               BF16 MFMA (residues widened to BF16, FP32
+                               accumulation folded mod m every 512 k, so
+                               exact): the same bits, at the BF16 rate      */
   CAPS_MODE_AUTO = 99       /* the planner (autotuner / learned model) picks */
 } caps_mode;
 
@@ -207,6 +211,7 @@ typedef struct caps_candidate {
 #define CAPS_MASK_FP64 (1 << CAPS_MODE_FP64)
 #define CAPS_MASK_OZAKI_INT8 (1 << CAPS_MODE_OZAKI_INT8)
 #define CAPS_MASK_CRT_INT8 (1 << CAPS_MODE_CRT_INT8)
+#define CAPS_MASK_CRT_BF16 (1 << CAPS_MODE_CRT_BF16)
 
 /* High-precision product used as the error reference (device pointers). */
 int caps_reference_product(const double *dA, const double *dB, double *dRef,

@@ -58,6 +58,7 @@ def _setup():
     (56, 3, 1, ["bbb", "ddd"], 12),    # odd leaves (m = 7)
     (256, 1, 1, ["b"], 14),            # 128 x 128 leaves: the fast GEMM path
     (512, 1, 1, ["b", "d"], 14),       # 256 x 256 leaves: the large-tile GEMM
+    (384, 1, 1, ["b", "d"], 14),       # 192 x 192 leaves: whole K slices, edge rows checked
 ])
 def test_crt_mode_bit_exact_vs_model(n, r, bs, patterns, moduli):
     A, B = realdata(n, 100 + n)
@@ -166,3 +167,40 @@ def test_crt_fused_levels_bit_exact(n, nrec, pattern):
     assert np.array_equal(fused, plain)
     if n <= 512:
         assert np.array_equal(oracle.from_caps(fused, n, nrec, 1), split.crt_mode(A, B, 14))
+
+
+BF16 = caps_amd.MODE_CRT_BF16
+
+
+@pytest.mark.parametrize("n,r,patterns", [(256, 1, ["b"]), (384, 1, ["b", "d"]), (56, 3, ["bbb"])])
+def test_crt_bf16_bit_exact_vs_model(n, r, patterns):
+    """BF16 leaf products (FP32 accumulation folded mod m): the exact model's
+    bits, on the fast, edge-checked and byte-checked GEMM paths."""
+    A, B = realdata(n, 300 + n)
+    want = split.crt_mode(A, B, 14)
+    a, b = oracle.to_caps(A, r, 1), oracle.to_caps(B, r, 1)
+    for pat in patterns:
+        got = oracle.from_caps(run(a, b, n, r, 1, pat, BF16, 14), n, r, 1)
+        assert np.array_equal(got, want), pat
+
+
+@pytest.mark.parametrize("n,r,pattern,cutoff", [(4096, 1, "b", 0), (2048, 2, "bb", 1),
+                                                (1536, 2, "bd", 0)])
+def test_crt_bf16_equals_int8_mode(n, r, pattern, cutoff):
+    """Large leaves (K up to 2048: several FP32 folds) and the quadtree
+    operands of a cutoff: the BF16 and INT8 leaf GEMMs give the same bits."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(n + cutoff)
+    u = torch.rand(2 * n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    e = torch.randint(-10, 11, (2 * n * n,), generator=g, device="cuda", dtype=torch.int32)
+    x = torch.ldexp(u, e.to(torch.float64))
+    dA, dB = x[: n * n].contiguous(), x[n * n:].contiguous()
+    d = caps_amd.MatDescriptor.one_device(n, r, 1)
+    outs = []
+    for mode in (CRT, BF16):
+        print(f"[gpu] crt n={n} r={r} pattern={pattern} cutoff={cutoff} mode={mode}", flush=True)
+        dC = torch.empty_like(dA)
+        caps_amd.multiply_device(dA, dB, dC, d, pattern, mode=mode, moduli=14, cutoff=cutoff)
+        torch.cuda.synchronize()
+        outs.append(dC.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
