@@ -11,8 +11,9 @@ Shapes: n = 256 round(exp(U[ln 512, ln 16384]) / 256); layout depth r = the
 largest with leaves n / 2^r >= 512 and n divisible by 2^r (at least 0); entries u 2^e with
 u uniform [-1, 1), e uniform integer in [-spread, spread], spread uniform in
 {0, ..., 20}.  Training and held-out shapes come from different seeds.
-Candidates: plain FP64 and the residue-domain mode (moduli 12..16), every
-pattern b^k d^(L-k) and every admissible cutoff.  Each shape's rows are
+Candidates: plain FP64 and the residue-domain mode (moduli 12..16, or the
+subset given by --crt-moduli), every pattern b^k d^(L-k) and every
+admissible cutoff.  Each shape's rows are
 written as soon as the shape is done, so a part can be resumed.
 """
 from __future__ import annotations
@@ -63,6 +64,9 @@ def main(argv=None) -> int:
     ap.add_argument("--reps", type=int, default=1)
     ap.add_argument("--budget-s", type=float, default=900.0,
                     help="stop after this many seconds (rows so far are kept)")
+    ap.add_argument("--crt-moduli", default="",
+                    help="comma-separated moduli counts of the residue-domain candidates to keep "
+                         "(default: all the library enumerates)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
@@ -93,6 +97,9 @@ def main(argv=None) -> int:
         Ref = torch.empty_like(A)
         caps_amd.reference_product(A, B, Ref, desc)
         cands = caps_amd.candidates(desc, mask)
+        if a.crt_moduli:
+            keep = {int(x) for x in a.crt_moduli.split(",")}
+            cands = [c for c in cands if c.mode != caps_amd.MODE_CRT_INT8 or c.moduli in keep]
         ts = time.time()
         for c in cands:
             caps_amd.measure(A, B, C, desc, c, Ref, a.reps)
