@@ -1,0 +1,77 @@
+"""The planner evaluation (caps_amd.planner_train.evaluate) on synthetic
+runs, with stand-in predictors: a perfect one has zero regret and no
+violations; one that ignores the error target violates it; the no-model
+default and the reference rule are scored from their own rows."""
+import numpy as np
+
+import caps_amd
+from caps_amd import planner_train as pt
+
+
+def _runs():
+    # 3 shapes x 4 candidates: (mode, moduli, cutoff, pattern, ms, err)
+    cands = [
+        (caps_amd.MODE_FP64, 0, 0, "dd", 10.0, 2e-14),          # reference rule, misses the bar
+        (caps_amd.MODE_FP64, 0, 0, "bb", 8.0, 3e-14),           # fast FP64, misses the bar
+        (caps_amd.MODE_CRT_INT8, 14, 0, "bb", 6.0, 5e-15),     # default
+        (caps_amd.MODE_CRT_INT8, 14, 1, "bb", 4.0, 4e-15),     # best feasible
+    ]
+    rows = {k: [] for k in ("shape", "n", "nrec", "spread", "features", "ms", "err", "mode",
+                            "moduli", "cutoff", "pattern")}
+    for s in range(3):
+        for i, (mode, mod, cut, pat, ms, err) in enumerate(cands):
+            rows["shape"].append(s)
+            rows["n"].append(1024)
+            rows["nrec"].append(2)
+            rows["spread"].append(0)
+            f = np.zeros(8, np.float32)
+            f[0] = i  # the stand-in predictors key on this
+            rows["features"].append(f)
+            rows["ms"].append(ms * (1 + s))
+            rows["err"].append(err)
+            rows["mode"].append(mode)
+            rows["moduli"].append(mod)
+            rows["cutoff"].append(cut)
+            rows["pattern"].append(pat)
+    return {k: np.array(v) for k, v in rows.items()}
+
+
+class _Stand:
+    """forward() returns normalised (log2 ms, log10 err) from a table keyed
+    by feature 0; with the identity normaliser below these are the values."""
+
+    def __init__(self, table):
+        self.table = table
+
+    def forward(self, X):
+        return np.array([self.table[int(round(x[0]))] for x in X], dtype=np.float32)
+
+
+NORM = np.concatenate([np.zeros(8), np.ones(8), np.zeros(2), np.ones(2)]).astype(np.float32)
+
+
+def test_perfect_predictor_has_zero_regret():
+    runs = _runs()
+    truth = {i: (np.log2(ms), np.log10(err)) for i, (_, _, _, _, ms, err) in enumerate([
+        (0, 0, 0, 0, 10.0, 2e-14), (0, 0, 0, 0, 8.0, 3e-14), (0, 0, 0, 0, 6.0, 5e-15),
+        (0, 0, 0, 0, 4.0, 4e-15)])}
+    rep = pt.evaluate(_Stand(truth), NORM, runs)
+    assert rep["model"]["mean_regret"] == 0.0 and rep["model"]["violations"] == 0
+    assert abs(rep["default"]["mean_regret"] - 0.5) < 1e-12 and rep["default"]["violations"] == 0
+    assert rep["reference"]["violations"] == 3
+    assert rep["shapes"] == 3 and rep["model_fallbacks_to_exhaustive"] == 0
+
+
+def test_predictor_blind_to_error_violates():
+    runs = _runs()
+    blind = {0: (3.0, -16.0), 1: (1.0, -16.0), 2: (2.6, -16.0), 3: (2.0, -16.0)}
+    rep = pt.evaluate(_Stand(blind), NORM, runs)
+    assert rep["model"]["violations"] == 3  # it picks the fast FP64 candidate
+
+
+def test_no_candidate_predicted_feasible_falls_back_to_exhaustive():
+    runs = _runs()
+    pessimist = {i: (1.0, -10.0) for i in range(4)}
+    rep = pt.evaluate(_Stand(pessimist), NORM, runs)
+    assert rep["model_fallbacks_to_exhaustive"] == 3
+    assert rep["model"]["mean_regret"] == 0.0
