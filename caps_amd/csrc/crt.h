@@ -35,6 +35,7 @@ struct CrtMats {
   double *pa;  // row partials (e, s) of A: [2^nrec * chunks][n] pairs
   double *pb;  // column partials of B: [2^nrec][n] pairs
   int n, nrec, bs, m, nmod, chunks;
+  int fused_b;  // B's split is fused with the top pre-additions (no rb planes)
 };
 
 // residue Winograd step: like EwPtrs with a plane stride per pointer

@@ -383,6 +383,7 @@ hipError_t launch_crt_step(int step, const CrtMats *M, const CrtMats &h, const O
         hipLaunchKernelGGL((crt_split_a_kernel<true>), ga, dim3(T), 0, s, M, kc);
       else
         hipLaunchKernelGGL((crt_split_a_kernel<false>), ga, dim3(T), 0, s, M, kc);
+      if (h.fused_b) return hipGetLastError();
       const long long mm = (long long)h.m * h.m;
       const bool fb = col && h.m % 16 == 0;
       dim3 gb(capped_blocks(fb ? mm / 16 : mm, leaves), leaves);

@@ -370,7 +370,8 @@ struct Builder {
     // plain sub-blocks passed by pointer: level-1 / level-2 operand -> quadrant
     const int quadA[7] = {0, 2, -1, -1, -1, -1, 3};  // T1 = A11, T2 = A12, T7 = A22
     const int quadB[7] = {0, 1, -1, -1, -1, 3, -1};  // S1 = B11, S2 = B21, S6 = B22
-    Mat LA = alloc_mats(40, q), LB = alloc_mats(40, q), P2 = alloc_mats(49, q);
+    const bool fused_b = crt && crt->fused_b;
+    Mat LA = alloc_mats(40, q), LB = alloc_mats(fused_b ? 49 : 40, q), P2 = alloc_mats(49, q);
     // the kernels address the 40 written operands of a side as out + k * step
     // (k counts the non-plain (i, j) in order) and product (i, j) as
     // q + (7 i + j) * step
@@ -395,16 +396,29 @@ struct Builder {
         } else {
           a = at(LA, (long long)ka++ * q);
         }
-        if (bb_plain(1, i) && bb_plain(1, j)) {
+        if (fused_b) {
+          b = at(LB, (long long)k * q);
+        } else if (bb_plain(1, i) && bb_plain(1, j)) {
           b = at(pb.B, (4 * quadB[i] + quadB[j]) * q);
         } else {
           b = at(LB, (long long)kb++ * q);
         }
         next[k] = {a, b, at(P2, k * q)};
       }
-    if (ka != 40 || kb != 40) broken = true;
+    if (ka != 40 || kb != (fused_b ? 0 : 40)) broken = true;
     if (residue) ensure_consts();
-    push(STEP_BB_PRE, residue, 1, q, 0, add_table(&pre, sizeof(pre)));
+    if (fused_b) {
+      BBSplitB sp{};
+      sp.B = crt->B;
+      sp.out = reinterpret_cast<int8_t *>(LB.p);
+      sp.sout = LB.ps;
+      sp.sb = crt->sb;
+      sp.sub = sub;
+      sp.m0 = crt->m;
+      sp.nrec2 = crt->nrec - 2;
+      push(STEP_BB_SPLIT_B, 1, 1, q, 0, add_table(&sp, sizeof(sp)));
+    }
+    push(STEP_BB_PRE, residue, 1, q, 0, add_table(&pre, sizeof(pre)), fused_b ? 1 : 0);
     rec(next, sub, nrec - 2, depth + 2);
     BBPost post{};
     post.q = P2.p;
@@ -453,6 +467,12 @@ struct Builder {
     ensure_consts();
     std::vector<Prob> top_batch;
     std::vector<size_t> mats;
+    // two fused 'b' levels at the top: the reconstruction is fused with the
+    // combinations; with an even leaf size B's split is fused with its
+    // pre-additions too (no residue planes of C and B).  A's split stays a
+    // pass of its own: its leaves are transposed to row-major on the way.
+    const bool top2 = items.size() == 1 && fusable(1, n, d.nrec, 0) && (d.bs == 1 || d.bs == m);
+    const bool fused_b = top2 && m % 2 == 0;
     for (const BatchItem *it : items) {
       CrtMats M{};
       M.A = it->A;
@@ -469,7 +489,10 @@ struct Builder {
       M.sb = reinterpret_cast<int *>(base + alloc(sizeof(int) * n));
       M.pa = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk * M.chunks));
       M.pb = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk));
-      Mat RA = alloc_mats(1, n2), RB = alloc_mats(1, n2), RC = alloc_mats(1, n2);
+      M.fused_b = fused_b;
+      Mat RA = alloc_mats(1, n2);
+      Mat RB = fused_b ? Mat{} : alloc_mats(1, n2);
+      Mat RC = top2 ? Mat{} : alloc_mats(1, n2);
       M.ra = reinterpret_cast<int8_t *>(RA.p);
       M.rb = reinterpret_cast<int8_t *>(RB.p);
       M.rc = reinterpret_cast<int8_t *>(RC.p);
@@ -483,8 +506,7 @@ struct Builder {
       top_batch.push_back({RA, RB, RC});
       mats.push_back(mt);
     }
-    if (items.size() == 1 && fusable(1, n, d.nrec, 0) && (d.bs == 1 || d.bs == m)) {
-      // two fused 'b' levels; the reconstruction is fused with the combinations
+    if (top2) {
       bfs2(top_batch[0], n, d.nrec, 0, &plan.crt_host.back());
       return;
     }
@@ -526,7 +548,10 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
                                st.n, plan.oz_moduli, st.vec2 != 0, kc, s);
     case STEP_BB_PRE:
       return launch_bb_pre(reinterpret_cast<const BBPre *>(tb + st.table), st.n, st.op != 0,
-                           plan.oz_moduli, kc, s);
+                           plan.oz_moduli, kc, s, st.vec2 != 0);
+    case STEP_BB_SPLIT_B:
+      return launch_bb_split_b(reinterpret_cast<const BBSplitB *>(tb + st.table), st.n, kc, s);
+
     case STEP_BB_POST:
       return launch_bb_post(reinterpret_cast<const BBPost *>(tb + st.table), st.n, st.op != 0,
                             st.op ? ozaki_npieces(plan.oz_moduli) : 0, kc, s);
@@ -553,7 +578,8 @@ int timer_of(const Step &st) {
     case STEP_OZ_SCALE:
     case STEP_OZ_SPLIT:
     case STEP_CRT_SCALE:
-    case STEP_CRT_SPLIT: return CAPS_TIMER_SPLIT;
+    case STEP_CRT_SPLIT:
+    case STEP_BB_SPLIT_B: return CAPS_TIMER_SPLIT;
     case STEP_OZ_RECON:
     case STEP_CRT_RECON: return CAPS_TIMER_RECON;
   }

@@ -44,6 +44,7 @@ enum StepKind : int {
   STEP_CRT_RECON = 10, // residue mode: C residues -> C
   STEP_BB_PRE = 11,    // two 'b' levels of pre-additions in one pass (op = residue)
   STEP_BB_POST = 12,   // two 'b' levels of combinations (residue: + reconstruction)
+  STEP_BB_SPLIT_B = 13,  // residue mode: B's split fused with its two top 'b' levels
 };
 
 struct Step {

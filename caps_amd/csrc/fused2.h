@@ -41,9 +41,26 @@ struct BBPost {
   int bs;
 };
 
-// FP64: q % 2 == 0; residue: q % 4 == 0
+// residue mode at the top of the recursion: the B side of the pre-additions
+// fused with B's split.  FP64 B (column-major leaves) in, the residues of
+// all 49 depth-2 operands out (the plain ones too).
+struct BBSplitB {
+  const double *B;        // 16 depth-2 sub-blocks of q elements, (a, b) at element (4a + b) * q
+  int8_t *out;            // operand (i, j) of modulus l at out + l * sout + (7 i + j) * q
+  long long sout;         // plane stride (bytes)
+  const int *sb;          // scale exponents of B's global columns
+  int sub;                // depth-2 sub-block size (n / 4)
+  int m0;                 // leaf size (even)
+  int nrec2;              // quadrant levels inside a depth-2 sub-block
+};
+
+// FP64: q % 2 == 0; residue: q % 4 == 0.  a_only (residue): side B is
+// produced by launch_bb_split_b instead.
 hipError_t launch_bb_pre(const BBPre *d_P, long long q, bool residue, int nmod,
-                         const OzakiConsts *d_consts, hipStream_t s);
+                         const OzakiConsts *d_consts, hipStream_t s, bool a_only = false);
+// q % 4 == 0, q < 2^31, m0 even
+hipError_t launch_bb_split_b(const BBSplitB *d_P, long long q, const OzakiConsts *d_consts,
+                             hipStream_t s);
 hipError_t launch_bb_post(const BBPost *d_P, long long q, bool residue, int npieces,
                           const OzakiConsts *d_consts, hipStream_t s);
 

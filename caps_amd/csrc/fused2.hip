@@ -24,6 +24,9 @@ namespace {
 
 constexpr int T = 256;
 
+// global-memory views (stores through them are global_*, not flat_*)
+typedef __attribute__((address_space(1))) unsigned short gushort;
+
 // level-1 operands at one sub-position from the four quadrant values (order
 // 11, 21, 12, 22) -> T1..T7 (A side) or S1..S7 (B side)
 template <typename V>
@@ -185,6 +188,72 @@ __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__
   }
 }
 
+// ------------------------------------------------ residue pre of B fused with the split
+// thread: 2 consecutive positions of one leaf column (m0 even).  The 16 FP64
+// inputs are scaled by their global columns' exponents and truncated once,
+// exactly as the separate split does; then for every modulus they are reduced
+// to symmetric residues and combined into all 49 depth-2 operands.  Per
+// position: 16 x 8 bytes in, 49 x nmod bytes out.
+__global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restrict__ P,
+                                                       long long q,
+                                                       const OzakiConsts *__restrict__ kc) {
+  const int nmod = kc->nmod;
+  const unsigned n2 = (unsigned)(q >> 1);
+  const unsigned m0 = (unsigned)P->m0, mm0 = m0 * m0;
+  const int sub = P->sub;
+  for (unsigned u = blockIdx.x * T + threadIdx.x; u < n2; u += gridDim.x * T) {
+    const unsigned p = 2 * u;
+    const unsigned L2 = p / mm0, e = p - L2 * mm0;
+    int bi, bj;
+    leaf_coords((int)L2, P->nrec2, bi, bj);
+    const int cj = bj * (int)m0 + (int)(e / m0);
+    int sc[4];  // exponent of column half (a >> 1, b >> 1) as index 2 (a >> 1) + (b >> 1)
+#pragma unroll
+    for (int h = 0; h < 4; h++) sc[h] = P->sb[(h >> 1) * 2 * sub + (h & 1) * sub + cj];
+    double x[16][2];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const double2 v = *reinterpret_cast<const double2 *>(P->B + (long long)k * q + p);
+      const int s = sc[2 * (k >> 3) + ((k & 3) >> 1)];
+      x[k][0] = trunc(ldexp(v.x, s));
+      x[k][1] = trunc(ldexp(v.y, s));
+    }
+    for (int l = 0; l < nmod; l++) {
+      const int mi = kc->m[l];
+      const double inv = kc->inv_m[l];
+      const float mf = (float)mi, invf = 1.0f / mf;
+      int8_t *ob = P->out + l * P->sout;
+      int l1[2][7][4];  // [position][level-1 operand i][sub-position b]
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          int qv[4], o[7];
+#pragma unroll
+          for (int a = 0; a < 4; a++) qv[a] = sym_residue(x[4 * a + b][h], mi, inv);
+          level_ops(1, qv, o);
+#pragma unroll
+          for (int i = 0; i < 7; i++) l1[h][i][b] = o[i];
+        }
+#pragma unroll
+      for (int i = 0; i < 7; i++) {
+        int o0[7], o1[7];
+        level_ops(1, l1[0][i], o0);
+        level_ops(1, l1[1][i], o1);
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+          // plain operands are input residues, already reduced
+          const bool plain = bb_plain(1, i) && bb_plain(1, j);
+          const int r0 = plain ? o0[j] : mod_small(o0[j], mf, invf);
+          const int r1 = plain ? o1[j] : mod_small(o1[j], mf, invf);
+          const unsigned v = (unsigned)(r0 & 0xff) | ((unsigned)(r1 & 0xff) << 8);
+          ((gushort *)(ob + (7 * i + j) * q))[u] = (unsigned short)v;
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- residue post + CRT
 // One block per PPOS consecutive positions of the depth-2 sub-blocks, two
 // phases through LDS:
@@ -194,6 +263,7 @@ __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__
 //      positions: CRT over the moduli from LDS, unscale, FP64 store.
 constexpr int PPOS = 128;
 constexpr int PG = PPOS / 4;  // 4-position words per block
+static_assert(T % PPOS == 0, "phase 2 maps a lane to one position");
 
 template <int NP>
 __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict__ P, long long q,
@@ -244,26 +314,29 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
 
   const int m0 = P->m0, sub = P->sub, nrec2 = P->nrec2, bs = P->bs;
   const long long mm0 = (long long)m0 * m0;
-  for (int pr = threadIdx.x; pr < 16 * PPOS; pr += T) {
-    const int o = pr / PPOS, pl = pr % PPOS;
-    const long long u = pos0 + pl;
-    if (u >= q) continue;
+  // lane -> one position pl (T is a multiple of PPOS) and outputs o = o0, o0 + T / PPOS, ...
+  const int pl = threadIdx.x % PPOS, o0 = threadIdx.x / PPOS;
+  const long long u = pos0 + pl;
+  if (u >= q) return;
+  // position u of a depth-2 sub-block: leaf L2 (quadtree of nrec2 levels), column-major leaf
+  const long long L2 = u / mm0, e = u - L2 * mm0;
+  const int li = (int)(e % m0), lj = (int)(e / m0);
+  int bi, bj;
+  leaf_coords((int)L2, nrec2, bi, bj);
+  const long long dst_in_sub = L2 * mm0 + leaf_elem_offset(m0, bs, li, lj);
+  const int ri = bi * m0 + li, cj = bj * m0 + lj;
+  const unsigned lds_shift = 8 * (pl & 3);
+  for (int o = o0; o < 16; o += T / PPOS) {
     double S[NP];
 #pragma unroll
     for (int p = 0; p < NP; p++) S[p] = 0.0;
     for (int l = 0; l < nmod; l++)
-      crt_acc<NP>(S, (int)__builtin_amdgcn_sbfe(R[l][o][pl >> 2], 8 * (pl & 3), 8), l, kc);
-    // position u of a depth-2 sub-block: leaf L2 (quadtree of nrec2 levels), column-major leaf
-    const long long L2 = u / mm0, e = u - L2 * mm0;
-    const int li = (int)(e % m0), lj = (int)(e / m0);
-    int bi, bj;
-    leaf_coords((int)L2, nrec2, bi, bj);
+      crt_acc<NP>(S, (int)__builtin_amdgcn_sbfe(R[l][o][pl >> 2], lds_shift, 8), l, kc);
     const int aa = o >> 2, bb = o & 3;
-    const int gi = (aa & 1) * 2 * sub + (bb & 1) * sub + bi * m0 + li;
-    const int gj = (aa >> 1) * 2 * sub + (bb >> 1) * sub + bj * m0 + lj;
+    const int gi = (aa & 1) * 2 * sub + (bb & 1) * sub + ri;
+    const int gj = (aa >> 1) * 2 * sub + (bb >> 1) * sub + cj;
     const double X = crt_finish<NP>(S, kc);
-    P->out_c[o * q + L2 * mm0 + leaf_elem_offset(m0, bs, li, lj)] =
-        ldexp(X, -(P->sa[gi] + P->sb[gj]));
+    P->out_c[o * q + dst_in_sub] = ldexp(X, -(P->sa[gi] + P->sb[gj]));
   }
 }
 
@@ -278,7 +351,7 @@ unsigned blocks_for(long long units, int per_grid) {
 }  // namespace
 
 hipError_t launch_bb_pre(const BBPre *P, long long q, bool residue, int nmod,
-                         const OzakiConsts *kc, hipStream_t s) {
+                         const OzakiConsts *kc, hipStream_t s, bool a_only) {
   if (!residue) {
     if (q % 2) return hipErrorInvalidValue;
     dim3 g(blocks_for(q / 2, 2), 2);
@@ -287,8 +360,15 @@ hipError_t launch_bb_pre(const BBPre *P, long long q, bool residue, int nmod,
     if (q % 4 || q >= (1LL << 31)) return hipErrorInvalidValue;
     dim3 g(blocks_for(q / 2, 2 * nmod), nmod);
     hipLaunchKernelGGL(bb_pre_res_kernel<0>, g, dim3(T), 0, s, P, q, kc);
-    hipLaunchKernelGGL(bb_pre_res_kernel<1>, g, dim3(T), 0, s, P, q, kc);
+    if (!a_only) hipLaunchKernelGGL(bb_pre_res_kernel<1>, g, dim3(T), 0, s, P, q, kc);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_bb_split_b(const BBSplitB *P, long long q, const OzakiConsts *kc,
+                             hipStream_t s) {
+  if (q % 4 || q >= (1LL << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bb_split_b_kernel, dim3(blocks_for(q / 2, 1)), dim3(T), 0, s, P, q, kc);
   return hipGetLastError();
 }
 

@@ -350,6 +350,7 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
 // group, chunks lk and lk' of one half-wave) hit distinct 16-byte slots.
 constexpr int JBM = 256, JBK = 128, JT = 512;
 constexpr int JSTAGE = JBM * JBK;  // 32 KB per operand per stage
+typedef __attribute__((address_space(1))) unsigned gu32;  // stores as global_*, not flat_*
 
 __device__ __forceinline__ int jswz(int r) { return ((r >> 2) & 3) << 1; }
 
@@ -405,11 +406,13 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
     const int c14 = kc->c14[l];
     const int row0 = ti * JBM, col0 = tj * JBM;
 
+    // small K: the sums start at 128 for the one-pass reduction res_byte128
+    const int bias = small_k ? 128 : 0;
     v4i acc[8][4];
 #pragma unroll
     for (int r = 0; r < 8; r++)
 #pragma unroll
-      for (int c = 0; c < 4; c++) acc[r][c] = v4i{0, 0, 0, 0};
+      for (int c = 0; c < 4; c++) acc[r][c] = v4i{bias, bias, bias, bias};
 
     const long long mm0 = (long long)m0 * m0;
     const long long arow = (long long)(row0 % m0) * m0, bcol = (long long)(col0 % m0) * m0;
@@ -451,25 +454,33 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
       __syncthreads();
     }
 
-    // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v
-    int8_t *Cl = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0;
+    // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v; the
+    // lane's word of block (r, c) sits at Cb + 16 c m0 + 16 r
+    int8_t *Cb = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0 +
+                 (long long)((col0 % m0) + wn * 64 + l16) * m0 + (row0 % m0) + wm * 128 + 4 * lk;
+    if (small_k) {
+      const int c14m = c14 - 16384;
+      const float cl = 0.5f - 128.0f * invf;
 #pragma unroll
-    for (int r = 0; r < 8; r++)
+      for (int r = 0; r < 8; r++)
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const int row = (row0 % m0) + wm * 128 + r * 16 + 4 * lk;
-        const int col = (col0 % m0) + wn * 64 + c * 16 + l16;
-        unsigned v = 0;
-        if (small_k) {
+        for (int c = 0; c < 4; c++) {
+          unsigned v = 0;
 #pragma unroll
-          for (int q = 0; q < 4; q++)
-            v |= (unsigned)(mod_acc(acc[r][c][q], c14, mf, invf) & 0xff) << (8 * q);
-        } else {
+          for (int q = 0; q < 4; q++) v = res_byte128(acc[r][c][q], c14m, mf, invf, cl, q, v);
+          *(gu32 *)(Cb + (long long)(16 * c) * m0 + 16 * r) = v ^ 0x80808080u;
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          unsigned v = 0;
 #pragma unroll
           for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
+          *reinterpret_cast<unsigned *>(Cb + (long long)(16 * c) * m0 + 16 * r) = v;
         }
-        *reinterpret_cast<unsigned *>(Cl + (long long)col * m0 + row) = v;
-      }
+    }
   }
 }
 

@@ -61,18 +61,15 @@ __device__ __forceinline__ int scale_exponent(int e, double s, double L) {
   return t;
 }
 
-// symmetric residue of the FP64 integer x (|x| < 2^60) modulo mi
+// forward declaration (defined below)
+__device__ __forceinline__ int mod_small(int x, float mf, float inv);
+
+// symmetric residue of the FP64 integer x (|x| <= 2^63) modulo mi: the FP64
+// quotient estimate x * inv is within |x / mi| 2^-52 <= 11 of x / mi, so r =
+// x - q mi is an integer with |r| < 12 mi, exact in one fma; mod_small finishes
 __device__ __forceinline__ int sym_residue(double x, int mi, double inv) {
-  // |x| <= 2^63: q is within 8 of x/m, so r = x - q m is a small integer
-  // (|r| < 9m, exact in one fma); a second exact step brings it to |r| <= m
-  const double q = rint(x * inv);
-  double r = fma(-q, (double)mi, x);
-  r = fma(-rint(r * inv), (double)mi, r);
-  int ri = (int)r;
-  const int hi = (mi - 1) >> 1, lo = -(mi >> 1);
-  ri = ri > hi ? ri - mi : ri;
-  ri = ri < lo ? ri + mi : ri;
-  return ri;
+  const double r = fma(-rint(x * inv), (double)mi, x);
+  return mod_small((int)r, (float)mi, 1.0f / (float)mi);
 }
 
 // residue of a small integer (|x| < 2^22) modulo mi in [-mi/2, mi/2]: the
@@ -92,6 +89,24 @@ __device__ __forceinline__ int mod_small(int x, float mf, float inv) {
 __device__ __forceinline__ int mod_acc(int x, int c14, float mf, float inv) {
   const int y = __mul24(x >> 14, c14) + (x & 16383);
   return mod_small(y, mf, inv);
+}
+
+// byte q of a packed INT8 residue word from an INT32 product sum that started
+// at 128: xb = x + 128, |x| <= 2^28 (K <= 2^14 products of INT8 residues).
+//   y = xb + (xb >> 14) (c14 - 2^14) = (xb >> 14) c14 + (xb & 16383) = xb
+//   (mod m), |y| < 2^21.1, in one mad24 (c14m = c14 - 2^14);
+//   k = floor(y / m + 1/2 - 128 / m) from fma(y, 1/m, cl) in FP32, cl = 1/2 -
+//   128/m: the error (< 1.2e-3) is below the distance 1/(2m) of that argument
+//   to an integer for odd m, and every step is exact for m = 256;
+//   y - k m = r + 128, r = x mod m in [-m/2, m/2): a byte value in [0, 255]
+//   that cvt_pk_u8 converts and places at byte q of `word`.  XOR 0x80 per
+//   byte then gives the INT8 residue r.
+__device__ __forceinline__ unsigned res_byte128(int xb, int c14m, float mf, float inv, float cl,
+                                                int q, unsigned word) {
+  const int y = __mul24(xb >> 14, c14m) + xb;
+  const float yf = (float)y;
+  const float k = floorf(fmaf(yf, inv, cl));
+  return __builtin_amdgcn_cvt_pk_u8_f32(fmaf(-k, mf, yf), q, word);
 }
 
 // symmetric residue of an INT32 product sum (|x| < 2^31) modulo mi

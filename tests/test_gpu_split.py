@@ -59,6 +59,8 @@ def _setup():
     (256, 1, 1, ["b"], 14),            # 128 x 128 leaves: the fast GEMM path
     (512, 1, 1, ["b", "d"], 14),       # 256 x 256 leaves: the large-tile GEMM
     (384, 1, 1, ["b", "d"], 14),       # 192 x 192 leaves: whole K slices, edge rows checked
+    (256, 2, 64, ["bb", "bd"], 14),    # bs = leaf size; 'bb': B's split fused with the adds
+    (232, 2, 1, ["bb"], 14),           # even leaves (m = 58), not a multiple of 4
 ])
 def test_crt_mode_bit_exact_vs_model(n, r, bs, patterns, moduli):
     A, B = realdata(n, 100 + n)
@@ -156,7 +158,8 @@ def test_crt_large_tile_gemm_bit_exact_vs_model():
         assert np.array_equal(got, want), (cutoff, pat, np.abs(got - want).max())
 
 
-@pytest.mark.parametrize("n,nrec,pattern", [(512, 2, "bb"), (1024, 3, "bbd"), (256, 3, "bbb")])
+@pytest.mark.parametrize("n,nrec,pattern", [(512, 2, "bb"), (1024, 3, "bbd"), (256, 3, "bbb"),
+                                             (56, 3, "bbb"), (512, 2, "bd")])
 def test_crt_fused_levels_bit_exact(n, nrec, pattern):
     """Fused two-level residue passes with the reconstruction folded into the
     combinations: same bits as the unfused residue recursion and the model."""
