@@ -13,7 +13,9 @@ u uniform [-1, 1), e uniform integer in [-spread, spread], spread uniform in
 {0, ..., 20}.  Training and held-out shapes come from different seeds.
 Candidates: plain FP64 and the residue-domain mode (moduli 12..16, or the
 subset given by --crt-moduli), every pattern b^k d^(L-k) and every
-admissible cutoff.  Each shape's rows are
+admissible cutoff; a candidate whose plan does not fit the device's free
+memory (deep all-'b' plans at the largest sizes) is skipped, as the
+exhaustive autotuner skips it.  Each shape's rows are
 written as soon as the shape is done, so a part can be resumed.
 """
 from __future__ import annotations
@@ -101,8 +103,17 @@ def main(argv=None) -> int:
             keep = {int(x) for x in a.crt_moduli.split(",")}
             cands = [c for c in cands if c.mode != caps_amd.MODE_CRT_INT8 or c.moduli in keep]
         ts = time.time()
+        skipped = 0
         for c in cands:
-            caps_amd.measure(A, B, C, desc, c, Ref, a.reps)
+            try:
+                caps_amd.measure(A, B, C, desc, c, Ref, a.reps)
+            except caps_amd.CapsError as e:
+                if e.code != 3:  # only a plan that does not fit the device is skipped
+                    raise
+                skipped += 1
+                caps_amd.release()
+                torch.cuda.empty_cache()
+                continue
             rows["shape"].append(i)
             rows["n"].append(n)
             rows["nrec"].append(r)
@@ -119,7 +130,8 @@ def main(argv=None) -> int:
         torch.cuda.empty_cache()
         np.savez(a.out, **{k: np.array(v) for k, v in rows.items()})
         print(f"[planner_data] {a.split} shape {i}: n={n} nrec={r} spread={spread} "
-              f"{len(cands)} candidates in {time.time() - ts:.1f} s (rows {len(rows['ms'])})",
+              f"{len(cands) - skipped} candidates in {time.time() - ts:.1f} s, {skipped} too large "
+              f"for the device (rows {len(rows['ms'])})",
               flush=True)
     return 0
 
