@@ -190,7 +190,11 @@ constexpr int ISTR = IBK + 16;  // LDS row stride (bytes): b128 reads of 16 lane
 // (k contiguous), 64 k each, 16 bytes per access.  Ab / Bb point at the
 // slice's first element, ld is the row (A) / column (B) stride.  CHECK 0:
 // none; 1: every byte against m; 2: rows / columns against m (m % 64 == 0,
-// so K slices are whole).  Checks use the slice origin; plain operands only.
+// so K slices are whole); 3: rows / columns and 16-byte K chunks against m
+// (m % 16 == 0, so a chunk is wholly inside or outside).  Checks use the
+// slice origin; plain operands only.
+typedef __attribute__((address_space(1))) unsigned gu32;  // stores as global_*, not flat_*
+
 template <int CHECK>
 __device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ab,
                                          const int8_t *__restrict__ Bb, long long ld, int m,
@@ -208,6 +212,12 @@ __device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ab,
       const v4u z = {0u, 0u, 0u, 0u};
       ga[it] = row0 + r < m ? *reinterpret_cast<const v4u *>(Ab + r * ld + kq) : z;
       gb[it] = col0 + r < m ? *reinterpret_cast<const v4u *>(Bb + r * ld + kq) : z;
+    } else if (CHECK == 3) {
+      // 16-byte chunks beyond K, rows of A / columns of B beyond m read as zero
+      const v4u z = {0u, 0u, 0u, 0u};
+      const bool kin = k0 + kq < m;
+      ga[it] = kin && row0 + r < m ? *reinterpret_cast<const v4u *>(Ab + r * ld + kq) : z;
+      gb[it] = kin && col0 + r < m ? *reinterpret_cast<const v4u *>(Bb + r * ld + kq) : z;
     } else {
       unsigned wa[4], wb[4];
 #pragma unroll
@@ -268,14 +278,17 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
     const int8_t *__restrict__ Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major
     int8_t *__restrict__ Cp = tab[leaf].c + l * tab[leaf].sc;
     const int mi = kc->m[l];
-    const float invf = 1.0f / (float)mi;
+    const float mf = (float)mi, invf = 1.0f / mf;
     const int row0 = ti * IBM, col0 = tj * IBN;
 
+    // small K: the sums start at 128 for the one-pass reduction res_byte128
+    const bool small_k = m <= 16384;
+    const int bias = small_k ? 128 : 0;
     v4i acc[4][4];
 #pragma unroll
     for (int r = 0; r < 4; r++)
 #pragma unroll
-      for (int c = 0; c < 4; c++) acc[r][c] = v4i{0, 0, 0, 0};
+      for (int c = 0; c < 4; c++) acc[r][c] = v4i{bias, bias, bias, bias};
 
     v4u ga[2], gb[2];
     const long long mm0 = (long long)m0 * m0;
@@ -319,15 +332,26 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
         const int col = col0 + wn * 64 + c * 16 + l16;
         int8_t *dst = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0 +
                       (long long)(col % m0) * m0 + (row % m0);
-        if (CHECK == 0) {
-          unsigned v = 0;
+        unsigned v = 0;
+        if (small_k) {
+          const int c14m = kc->c14[l] - 16384;
+          const float cl = 0.5f - 128.0f * invf;
+#pragma unroll
+          for (int q = 0; q < 4; q++) v = res_byte128(acc[r][c][q], c14m, mf, invf, cl, q, v);
+          v ^= 0x80808080u;
+        } else {
 #pragma unroll
           for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
-          *reinterpret_cast<unsigned *>(dst) = v;
+        }
+        if (CHECK == 0) {
+          *(gu32 *)dst = v;
+        } else if (CHECK >= 2) {
+          // m % 16 == 0: the 4 rows of the word are all inside or all outside
+          if (col < m && row < m) *(gu32 *)dst = v;
         } else if (col < m) {
 #pragma unroll
           for (int q = 0; q < 4; q++)
-            if (row + q < m) dst[q] = (int8_t)mod_i32(acc[r][c][q], mi, invf);
+            if (row + q < m) dst[q] = (int8_t)(v >> (8 * q));
         }
       }
   }
@@ -350,7 +374,6 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
 // group, chunks lk and lk' of one half-wave) hit distinct 16-byte slots.
 constexpr int JBM = 256, JBK = 128, JT = 512;
 constexpr int JSTAGE = JBM * JBK;  // 32 KB per operand per stage
-typedef __attribute__((address_space(1))) unsigned gu32;  // stores as global_*, not flat_*
 
 __device__ __forceinline__ int jswz(int r) { return ((r >> 2) & 3) << 1; }
 
@@ -629,6 +652,14 @@ __global__ __launch_bounds__(T256) void oz_bgemm_kernel(const RGemm *__restrict_
 #pragma unroll
           for (int q = 0; q < 4; q++) v |= (unsigned)(res[q] & 0xff) << (8 * q);
           *reinterpret_cast<unsigned *>(dst) = v;
+        } else if (CHECK >= 2) {
+          // m % 16 == 0: the 4 rows of the word are all inside or all outside
+          if (col < m && row < m) {
+            unsigned v = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) v |= (unsigned)(res[q] & 0xff) << (8 * q);
+            *(gu32 *)dst = v;
+          }
         } else if (col < m) {
 #pragma unroll
           for (int q = 0; q < 4; q++)
@@ -716,6 +747,9 @@ hipError_t launch_residue_gemm_bf16(const RGemm *tab, int count, int m, int leve
   else if (m % IBK == 0)
     hipLaunchKernelGGL((oz_bgemm_kernel<2>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
                        m0, levels);
+  else if (m % 16 == 0)
+    hipLaunchKernelGGL((oz_bgemm_kernel<3>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
+                       m0, levels);
   else
     hipLaunchKernelGGL((oz_bgemm_kernel<1>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
                        m0, levels);
@@ -742,7 +776,8 @@ hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, i
   if (!ok) return hipErrorInvalidValue;
   const int G = persistent_grid(2, device_cu_count(), map.total);
   // the fast path reads 16-byte rows: leaves of a multiple of 128; with m %
-  // 64 == 0 the K slices are whole and only edge rows / columns are checked
+  // 64 == 0 the K slices are whole and only edge rows / columns are checked;
+  // with m % 16 == 0 whole 16-byte K chunks are checked too; else every byte
   const bool fast = (m0 % IBM == 0);
   if (!fast && levels > 0) return hipErrorInvalidValue;
   if (fast)
@@ -750,6 +785,9 @@ hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, i
                        m0, levels);
   else if (m % IBK == 0)
     hipLaunchKernelGGL((oz_igemm_kernel<2>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
+                       m0, levels);
+  else if (m % 16 == 0)
+    hipLaunchKernelGGL((oz_igemm_kernel<3>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
                        m0, levels);
   else
     hipLaunchKernelGGL((oz_igemm_kernel<1>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,

@@ -61,6 +61,8 @@ def _setup():
     (384, 1, 1, ["b", "d"], 14),       # 192 x 192 leaves: whole K slices, edge rows checked
     (256, 2, 64, ["bb", "bd"], 14),    # bs = leaf size; 'bb': B's split fused with the adds
     (232, 2, 1, ["bb"], 14),           # even leaves (m = 58), not a multiple of 4
+    (160, 1, 1, ["b", "d"], 14),       # m = 80: 16-byte K chunks checked
+    (2 * 544, 1, 1, ["b"], 14),        # m = 544 (17 x 32), as at n = 4352, r = 3
 ])
 def test_crt_mode_bit_exact_vs_model(n, r, bs, patterns, moduli):
     A, B = realdata(n, 100 + n)
@@ -175,7 +177,8 @@ def test_crt_fused_levels_bit_exact(n, nrec, pattern):
 BF16 = caps_amd.MODE_CRT_BF16
 
 
-@pytest.mark.parametrize("n,r,patterns", [(256, 1, ["b"]), (384, 1, ["b", "d"]), (56, 3, ["bbb"])])
+@pytest.mark.parametrize("n,r,patterns", [(256, 1, ["b"]), (384, 1, ["b", "d"]), (56, 3, ["bbb"]),
+                                          (224, 1, ["b"])])
 def test_crt_bf16_bit_exact_vs_model(n, r, patterns):
     """BF16 leaf products (FP32 accumulation folded mod m): the exact model's
     bits, on the fast, edge-checked and byte-checked GEMM paths."""
