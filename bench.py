@@ -240,6 +240,37 @@ def batch_workload(args, torch, caps_amd):
     print(json.dumps(result), flush=True)
 
 
+def fp64_bdd_workload(args, torch, caps_amd, oracle, n=7168, r=3, pat="bdd"):
+    """Plain FP64 Strassen at n=7168, r=3, 'bdd' (the BFS level as 7 batched
+    sub-products, then two recursive levels, FP64 MFMA leaves of 896^2), entries
+    uniform [-1, 1): time per multiply, effective GFLOP/s and the normwise error
+    of 16 sampled full rows against the Dot2 product."""
+    log(f"[bench] plain FP64 n={n} nrec={r} pattern '{pat}'")
+    desc = caps_amd.MatDescriptor.one_device(n, r, 1)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2024)
+    A = torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    B = torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    C = torch.empty_like(A)
+
+    def run():
+        caps_amd.multiply_device(A, B, C, desc, pat, mode=caps_amd.MODE_FP64)
+
+    t = time_steps(torch, run, args.steps, args.warmup)
+    out = {"ms_per_step": round(t * 1e3, 4), "gflops": round(2.0 * n ** 3 / t / 1e9, 2)}
+    log(f"[bench] plain FP64 {pat}: {t * 1e3:.3f} ms/step")
+    if not args.no_check:
+        rows = np.sort(np.random.default_rng(7).choice(n, 16, replace=False))
+        out["error"] = sampled_error(oracle, oracle.from_caps(A.cpu().numpy(), n, r, 1),
+                                     oracle.from_caps(B.cpu().numpy(), n, r, 1),
+                                     oracle.from_caps(C.cpu().numpy(), n, r, 1), rows)
+        out["error_metric"] = "max|C-C*|/max|C*| over 16 sampled full rows, C* = Dot2"
+        log(f"[bench] plain FP64 {pat} error {out['error']:.3e}")
+    del A, B, C
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=("split", "batch"), default="split",
@@ -253,6 +284,8 @@ def main():
     ap.add_argument("--moduli", type=int, default=14)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-fp64-bdd", action="store_true",
+                    help="skip the plain FP64 n=7168 r=3 'bdd' measurement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -399,6 +432,9 @@ def main():
                                             "C* = Dot2 compensated classical product")
         result["config"]["integer_pair_bit_exact"] = exact
         result["config"]["meets_error_bar"] = err_split <= 1e-14
+
+    if not args.no_fp64_bdd:
+        result["config"]["plain_fp64_n7168_bdd"] = fp64_bdd_workload(args, torch, caps_amd, oracle)
 
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle restatement)")

@@ -329,3 +329,32 @@ def test_deep_dfs_plan_runs_without_graph():
     got = run(a, b, n, r, 1, pat)
     want = oracle.multiply(a, b, n, r, 1, pat, 8)
     assert np.array_equal(got, want)
+
+
+def test_fp64_bdd_full_size():
+    """Plain FP64 Strassen at its full benchmark size: n=7168, r=3, 'bdd'
+    (the BFS level as 7 batched sub-products, FP64 MFMA leaves of 896^2),
+    entries uniform [-1, 1).  Size-independent properties: the batched level
+    gives the same bits as the recursive one ('ddd'), since the FP64 adds use
+    the same association; the normwise error against the 16-modulus residue
+    product (correctly rounded for these inputs) stays within 1e-13, a bound
+    that a dropped Winograd term or a swapped operand would exceed by far."""
+    n, r = 7168, 3
+    print(f"[gpu] full-size FP64 n={n} r={r} bdd / ddd", flush=True)
+    d = caps_amd.MatDescriptor.one_device(n, r, 1)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2024)
+    A = torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    B = torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    C1, C2, ref = torch.empty_like(A), torch.empty_like(A), torch.empty_like(A)
+    caps_amd.multiply_device(A, B, C1, d, "bdd", caps_amd.MODE_FP64)
+    caps_amd.multiply_device(A, B, C2, d, "ddd", caps_amd.MODE_FP64)
+    caps_amd.reference_product(A, B, ref, d)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    err = caps_amd.normwise_error(C1, ref, d)
+    print(f"[gpu] n={n} bdd normwise error {err:.3e}", flush=True)
+    assert 0 < err <= 1e-13
+    del A, B, C1, C2, ref
+    caps_amd.release()
+    torch.cuda.empty_cache()
