@@ -119,7 +119,8 @@ __device__ __forceinline__ int mod_i32(int x, int mi, float inv) {
   return r;
 }
 
-// S_p += r_l * W_l[p]: exact, every partial sum < 2^49
+// S_p += r_l * W_l[p] with pieces W_l[p] < 2^37: exact for |r_l| <= 2^11 (a
+// residue, or a sum congruent to it), every partial sum below 14 * 2^48 < 2^52
 template <int NP>
 __device__ __forceinline__ void crt_acc(double (&S)[NP], int r, int l,
                                         const OzakiConsts *__restrict__ kc) {
@@ -149,9 +150,11 @@ __device__ __forceinline__ double i128_to_double(__int128 x) {
 }
 
 // X = sum_p S_p 2^(37p) reduced to (-M/2, M/2): the quotient k = round(S/M)
-// from an FP64 estimate (exact unless X were within 2^-40 M of M/2, which the
-// scaling bound excludes), D_p = S_p - k M_p exact, X assembled exactly in
-// 128 bits (|X| < M/2 <= 2^127) and rounded once to FP64
+// from an FP64 estimate (|S| < 2^14.8 M, so the estimate of S/M is within
+// 2^-36 of it: exact unless X were within 2^-36 M of M/2, which the scaling
+// bound, |X| <= (1/2 - 2^-31) M, excludes), D_p = S_p - k M_p exact (|k| <
+// 2^15, |D_p| < 2^53), X assembled exactly in 128 bits (|X| < M/2 <= 2^127)
+// and rounded once to FP64
 template <int NP>
 __device__ __forceinline__ double crt_finish(const double (&S)[NP],
                                              const OzakiConsts *__restrict__ kc) {

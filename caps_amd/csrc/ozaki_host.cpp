@@ -149,7 +149,8 @@ bool ozaki_make_consts(int nmod, OzakiConsts &k) {
   k.invM = 1.0 / Md;
   k.log2M = std::log2(Md);
   // L^2 <= (M/2)(1 - 2^-30): keeps |X| at least 2^-31 M away from M/2, far
-  // beyond the 2^-40 M error of the quotient estimate in the reconstruction.
+  // beyond the 2^-36 M error of the quotient estimate in the reconstruction
+  // (inputs congruent to the residues with |r| <= 2^11).
   const double D = Md * 0.5 * (1.0 - std::ldexp(1.0, -30));
   k.L = std::sqrt(D) * (1.0 - std::ldexp(1.0, -40));
   return true;
