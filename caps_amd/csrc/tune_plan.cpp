@@ -72,8 +72,17 @@ void candidate_features(const caps_desc &d, const caps_candidate &c, double spre
   f[3] = (float)nd;
   f[4] = (float)c.cutoff;
   f[5] = (float)c.moduli;
-  f[6] = c.mode == CAPS_MODE_CRT_INT8 ? 1.f : 0.f;
-  f[7] = c.mode == CAPS_MODE_OZAKI_INT8 ? 1.f : 0.f;
+  // mode as one ordinal: 0 FP64, 1 residue (INT8), 2 leaf split, 3 residue (BF16)
+  f[6] = c.mode == CAPS_MODE_CRT_INT8     ? 1.f
+         : c.mode == CAPS_MODE_OZAKI_INT8 ? 2.f
+         : c.mode == CAPS_MODE_CRT_BF16   ? 3.f
+                                          : 0.f;
+  // power-of-two alignment of the layout's leaf, log2(gcd(leaf, 256)) / 8: it
+  // picks the leaf GEMM's path (256 tiles, 128 tiles, edge-checked variants)
+  const int leaf = d.lda >> d.nrec;
+  int a = 0;
+  while (a < 8 && leaf > 0 && (leaf >> a) % 2 == 0) a++;
+  f[7] = (float)a / 8.f;
 }
 
 namespace {

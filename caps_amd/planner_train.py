@@ -7,8 +7,8 @@ held-out shapes against the exhaustive autotuner.
 Runs come from caps_amd.planner_data.  The model is the library's 3-layer
 perceptron (8 -> 64 -> 64 -> 2, ReLU, FP32), trained by the library's own HIP
 forward / backward / Adam kernels (caps_mlp_train) on normalised features
-(log2 n, exponent spread, #b, #d, cutoff, moduli, residue mode, leaf-split
-mode) and targets (log2 ms, log10 normwise error).
+(log2 n, exponent spread, #b, #d, cutoff, moduli, mode, leaf alignment; see
+features_from_columns) and targets (log2 ms, log10 normwise error).
 
 Evaluation per held-out shape, with the error target 1e-14:
   exhaustive  the fastest measured candidate whose measured error meets the
@@ -37,10 +37,36 @@ TARGET = 1e-14
 MARGIN = 0.25  # decades, as in the library's planner
 
 
+_MODE_CODE = {caps_amd.MODE_FP64: 0, caps_amd.MODE_CRT_INT8: 1, caps_amd.MODE_OZAKI_INT8: 2,
+              caps_amd.MODE_CRT_BF16: 3}
+
+
+def features_from_columns(runs) -> np.ndarray:
+    """The library's 8 candidate features (caps_candidate_features), restated
+    from the recorded columns: log2 n, exponent spread, #b and #d levels above
+    the cutoff, cutoff, moduli, mode ordinal, log2(gcd(n / 2^nrec, 256)) / 8."""
+    N = len(runs["n"])
+    f = np.zeros((N, 8), np.float32)
+    for i in range(N):
+        n, nrec, cut = int(runs["n"][i]), int(runs["nrec"][i]), int(runs["cutoff"][i])
+        pat = str(runs["pattern"][i])[:nrec - cut]
+        nb = sum(ch in "bB" for ch in pat)
+        leaf = n >> nrec
+        a = 0
+        while a < 8 and leaf > 0 and (leaf >> a) % 2 == 0:
+            a += 1
+        f[i] = (np.float32(np.log2(float(n))), np.float32(float(runs["spread"][i])), nb,
+                len(pat) - nb, cut, int(runs["moduli"][i]), _MODE_CODE[int(runs["mode"][i])],
+                np.float32(a) / np.float32(8))
+    return f
+
+
 def load_runs(paths) -> dict:
     parts = [dict(np.load(p)) for p in paths]
     out = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
-    # shape ids are unique within a split; parts of one split never overlap
+    # shape ids are unique within a split; parts of one split never overlap.
+    # Features are recomputed with the library's current definition.
+    out["features"] = features_from_columns(out)
     return out
 
 

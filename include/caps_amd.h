@@ -236,7 +236,8 @@ int caps_tune_save(const char *path);
 int caps_tune_load(const char *path);
 /* The candidates caps_autotune would measure, and the 8 model features of a
  * candidate: log2 n, exponent spread, #b levels, #d levels, cutoff, moduli,
- * is-residue-mode, is-leaf-split-mode. */
+ * mode (0 FP64, 1 residue INT8, 2 leaf split, 3 residue BF16), and the
+ * leaf's power-of-two alignment log2(gcd(n / 2^nrec, 256)) / 8. */
 int caps_enumerate_candidates(const caps_desc *desc, int mode_mask, caps_candidate *out,
                               int max_out, int *n_out);
 int caps_candidate_features(const caps_desc *desc, const caps_candidate *c, double spread,

@@ -95,4 +95,7 @@ def test_candidate_enumeration(lib):
     small = caps_amd.candidates(D(392, 3, 7), caps_amd.MASK_FP64 | caps_amd.MASK_CRT_INT8)
     assert all(c.cutoff == 0 for c in small)
     f = caps_amd.candidate_features(D(8192, 2, 1), crt[0], 10.0)
-    assert list(f[:1]) == [13.0] and f[1] == 10.0 and f[6] == 1.0 and f[7] == 0.0
+    # mode ordinal 1 (residue INT8); leaves of 2048: alignment log2(256) / 8 = 1
+    assert list(f[:1]) == [13.0] and f[1] == 10.0 and f[6] == 1.0 and f[7] == 1.0
+    f = caps_amd.candidate_features(D(392, 3, 7), small[0], 0.0)
+    assert f[6] == 0.0 and f[7] == 0.0  # FP64; leaves of 49 are odd

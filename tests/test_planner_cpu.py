@@ -75,3 +75,26 @@ def test_no_candidate_predicted_feasible_falls_back_to_exhaustive():
     rep = pt.evaluate(_Stand(pessimist), NORM, runs)
     assert rep["model_fallbacks_to_exhaustive"] == 3
     assert rep["model"]["mean_regret"] == 0.0
+
+
+def test_feature_restatement_matches_library():
+    """planner_train.features_from_columns equals the library's
+    caps_candidate_features on every candidate of a range of layouts."""
+    rows = {k: [] for k in ("n", "nrec", "spread", "cutoff", "pattern", "moduli", "mode")}
+    want = []
+    mask = caps_amd.MASK_FP64 | caps_amd.MASK_CRT_INT8 | caps_amd.MASK_OZAKI_INT8
+    for n, r in [(512, 0), (1024, 1), (2304, 2), (4352, 3), (6912, 4), (8192, 2), (12800, 4),
+                 (96, 1), (56, 3)]:
+        d = caps_amd.MatDescriptor.one_device(n, r, 1)
+        for c in caps_amd.candidates(d, mask):
+            for spread in (0.0, 7.0):
+                want.append(caps_amd.candidate_features(d, c, spread))
+                for k, v in (("n", n), ("nrec", r), ("spread", spread), ("cutoff", c.cutoff),
+                             ("pattern", c.pattern.decode()), ("moduli", c.moduli),
+                             ("mode", c.mode)):
+                    rows[k].append(v)
+    runs = {k: np.array(v) for k, v in rows.items()}
+    got = pt.features_from_columns(runs)
+    assert np.array_equal(got, np.array(want, dtype=np.float32))
+    # leaves 512 (256-aligned), 576, 544, 432 (64-, 32-, 16-aligned), 7 (odd)
+    assert set(np.unique(got[:, 7]).tolist()) == {1.0, 0.75, 0.625, 0.5, 0.0}
