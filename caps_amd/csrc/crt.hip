@@ -144,6 +144,7 @@ __global__ __launch_bounds__(T) void crt_split_a_kernel(const CrtMats *__restric
   for (int l = 0; l < nmod; l++) {
     const int mi = kc->m[l];
     const double inv = kc->inv_m[l];
+    const int c22 = kc->c22[l];
     if (FAST) {
       unsigned w[4];
 #pragma unroll
@@ -151,14 +152,14 @@ __global__ __launch_bounds__(T) void crt_split_a_kernel(const CrtMats *__restric
         unsigned v = 0;
 #pragma unroll
         for (int b = 0; b < 4; b++)
-          v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv) & 0xff) << (8 * b);
+          v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv, c22) & 0xff) << (8 * b);
         w[q] = v;
       }
       *reinterpret_cast<uint4 *>(dst + l * plane) = make_uint4(w[0], w[1], w[2], w[3]);
     } else {
 #pragma unroll
       for (int c = 0; c < 16; c++)
-        if (k0 + kq + c < m) dst[l * plane + c] = (int8_t)sym_residue(x[c], mi, inv);
+        if (k0 + kq + c < m) dst[l * plane + c] = (int8_t)sym_residue(x[c], mi, inv, c22);
     }
   }
 }
@@ -192,13 +193,14 @@ __global__ __launch_bounds__(T) void crt_split_b_kernel(const CrtMats *__restric
       for (int l = 0; l < nmod; l++) {
         const int mi = kc->m[l];
         const double inv = kc->inv_m[l];
+        const int c22 = kc->c22[l];
         unsigned w[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           unsigned v = 0;
 #pragma unroll
           for (int b = 0; b < 4; b++)
-            v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv) & 0xff) << (8 * b);
+            v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv, c22) & 0xff) << (8 * b);
           w[q] = v;
         }
         *reinterpret_cast<uint4 *>(M->rb + l * plane + lbase + e0) =
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(T) void crt_split_b_kernel(const CrtMats *__restric
       const double x = trunc(ldexp(M->B[lbase + leaf_elem_offset(m, bs, li, lj)],
                                    M->sb[bj * m + lj]));
       for (int l = 0; l < nmod; l++)
-        M->rb[l * plane + lbase + u] = (int8_t)sym_residue(x, kc->m[l], kc->inv_m[l]);
+        M->rb[l * plane + lbase + u] = (int8_t)sym_residue(x, kc->m[l], kc->inv_m[l], kc->c22[l]);
     }
   }
 }

@@ -140,13 +140,12 @@ __global__ __launch_bounds__(T) void bb_post_f64_kernel(const BBPost *__restrict
 // ---------------------------------------------------------------- residue pre
 // thread: 2 consecutive positions (2-byte accesses); grid.y: modulus; one
 // instantiation per side.  Depth-2 values are sums of at most 16 residues:
-// |x| <= 2048.  Offsets within a plane are 32-bit (q < 2^31).
+// |x| <= 2048, reduced by mod_k.  Offsets within a plane are 32-bit (q < 2^31).
 template <int SIDE>
 __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__ P, long long q,
                                                        const OzakiConsts *__restrict__ kc) {
   const int l = blockIdx.y;
-  const int mi = kc->m[l];
-  const float mf = (float)mi, inv = 1.0f / (float)mi;
+  const int mi = kc->m[l], c22 = kc->c22[l];
   const int8_t *in = reinterpret_cast<const int8_t *>(P->in[SIDE]) + l * P->sin[SIDE];
   char *obase = reinterpret_cast<char *>(P->out[SIDE]) + l * P->sout[SIDE];
   const long long step = P->step;
@@ -179,8 +178,8 @@ __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__
 #pragma unroll
       for (int j = 0; j < 7; j++) {
         if (bb_plain(SIDE, i) && bb_plain(SIDE, j)) continue;
-        const unsigned v = (unsigned)(mod_small(o0[j], mf, inv) & 0xff) |
-                           ((unsigned)(mod_small(o1[j], mf, inv) & 0xff) << 8);
+        const unsigned v = (unsigned)(mod_k(o0[j], c22, mi) & 0xff) |
+                           ((unsigned)(mod_k(o1[j], c22, mi) & 0xff) << 8);
         *reinterpret_cast<unsigned short *>(obase + k * step + 2 * u) = (unsigned short)v;
         k++;
       }
@@ -221,7 +220,7 @@ __global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restric
     for (int l = 0; l < nmod; l++) {
       const int mi = kc->m[l];
       const double inv = kc->inv_m[l];
-      const float mf = (float)mi, invf = 1.0f / mf;
+      const int c22 = kc->c22[l];
       int8_t *ob = P->out + l * P->sout;
       int l1[2][7][4];  // [position][level-1 operand i][sub-position b]
 #pragma unroll
@@ -230,7 +229,7 @@ __global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restric
         for (int b = 0; b < 4; b++) {
           int qv[4], o[7];
 #pragma unroll
-          for (int a = 0; a < 4; a++) qv[a] = sym_residue(x[4 * a + b][h], mi, inv);
+          for (int a = 0; a < 4; a++) qv[a] = sym_residue(x[4 * a + b][h], mi, inv, c22);
           level_ops(1, qv, o);
 #pragma unroll
           for (int i = 0; i < 7; i++) l1[h][i][b] = o[i];
@@ -244,8 +243,8 @@ __global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restric
         for (int j = 0; j < 7; j++) {
           // plain operands are input residues, already reduced
           const bool plain = bb_plain(1, i) && bb_plain(1, j);
-          const int r0 = plain ? o0[j] : mod_small(o0[j], mf, invf);
-          const int r1 = plain ? o1[j] : mod_small(o1[j], mf, invf);
+          const int r0 = plain ? o0[j] : mod_k(o0[j], c22, mi);
+          const int r1 = plain ? o1[j] : mod_k(o1[j], c22, mi);
           const unsigned v = (unsigned)(r0 & 0xff) | ((unsigned)(r1 & 0xff) << 8);
           ((gushort *)(ob + (7 * i + j) * q))[u] = (unsigned short)v;
         }
@@ -275,8 +274,7 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
     const int g = unit % PG, l = unit / PG;
     const long long u4 = pos0 / 4 + g;
     if (4 * u4 >= q) continue;  // q % 4 == 0
-    const int mi = kc->m[l];
-    const float mf = (float)mi, inv = 1.0f / mf;
+    const int mi = kc->m[l], c22 = kc->c22[l];
     const char *qb = reinterpret_cast<const char *>(P->q) + l * P->sq;
     unsigned w[49];
 #pragma unroll
@@ -304,7 +302,7 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
         Ew<EW_BFS_POST>::apply(qv, c);  // |.| <= 2048
 #pragma unroll
         for (int aa = 0; aa < 4; aa++)
-          out[4 * aa + bb] |= (unsigned)(mod_small(c[aa], mf, inv) & 0xff) << (8 * byte);
+          out[4 * aa + bb] |= (unsigned)(mod_k(c[aa], c22, mi) & 0xff) << (8 * byte);
       }
     }
 #pragma unroll

@@ -37,6 +37,7 @@ struct OzakiConsts {
   int npieces;
   int m[OZ_MAX_MODULI];
   int c14[OZ_MAX_MODULI];                  // 2^14 mod m_l, symmetric (|c14| <= m_l / 2)
+  int c22[OZ_MAX_MODULI];                  // round(2^22 / m_l): quotients of small sums (mod_k)
   double inv_m[OZ_MAX_MODULI];
   double w[OZ_MAX_MODULI][OZ_MAX_PIECES];  // CRT weight W_l in base 2^37 digits
   double Mp[OZ_MAX_PIECES];                // M in base 2^37 digits

@@ -105,6 +105,7 @@ __global__ __launch_bounds__(T256) void oz_split_a_kernel(const OzLeaf *__restri
   for (int l = 0; l < nmod; l++) {
     const int mi = kc->m[l];
     const double inv = kc->inv_m[l];
+    const int c22 = kc->c22[l];
     if (!CHECK) {
       unsigned w[4];
 #pragma unroll
@@ -112,14 +113,14 @@ __global__ __launch_bounds__(T256) void oz_split_a_kernel(const OzLeaf *__restri
         unsigned v = 0;
 #pragma unroll
         for (int b = 0; b < 4; b++)
-          v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv) & 0xff) << (8 * b);
+          v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv, c22) & 0xff) << (8 * b);
         w[q] = v;
       }
       *reinterpret_cast<uint4 *>(base + l * plane) = make_uint4(w[0], w[1], w[2], w[3]);
     } else {
 #pragma unroll
       for (int c = 0; c < 16; c++)
-        if (k0 + kq + c < m) base[l * plane + c] = (int8_t)sym_residue(x[c], mi, inv);
+        if (k0 + kq + c < m) base[l * plane + c] = (int8_t)sym_residue(x[c], mi, inv, c22);
     }
   }
 }
@@ -150,13 +151,14 @@ __global__ __launch_bounds__(T256) void oz_split_b_kernel(const OzLeaf *__restri
       for (int l = 0; l < nmod; l++) {
         const int mi = kc->m[l];
         const double inv = kc->inv_m[l];
+        const int c22 = kc->c22[l];
         unsigned w[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           unsigned v = 0;
 #pragma unroll
           for (int b = 0; b < 4; b++)
-            v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv) & 0xff) << (8 * b);
+            v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv, c22) & 0xff) << (8 * b);
           w[q] = v;
         }
         *reinterpret_cast<uint4 *>(L.rb + l * plane + e0) = make_uint4(w[0], w[1], w[2], w[3]);
@@ -171,9 +173,10 @@ __global__ __launch_bounds__(T256) void oz_split_b_kernel(const OzLeaf *__restri
       for (int l = 0; l < nmod; l++) {
         const int mi = kc->m[l];
         const double inv = kc->inv_m[l];
+        const int c22 = kc->c22[l];
 #pragma unroll
         for (int c = 0; c < 16; c++)
-          if (e0 + c < plane) L.rb[l * plane + e0 + c] = (int8_t)sym_residue(x[c], mi, inv);
+          if (e0 + c < plane) L.rb[l * plane + e0 + c] = (int8_t)sym_residue(x[c], mi, inv, c22);
       }
     }
   }

@@ -61,15 +61,25 @@ __device__ __forceinline__ int scale_exponent(int e, double s, double L) {
   return t;
 }
 
-// forward declaration (defined below)
-__device__ __forceinline__ int mod_small(int x, float mf, float inv);
+// residue of a small integer (|x| <= 8192) modulo mi (2 <= mi <= 256) in
+// [-mi/2, mi/2], in integer arithmetic: with c22 = round(2^22 / mi),
+// q = (x c22 + 2^21) >> 22 rounds x c22 / 2^22, which is within 8192 * 2^-23
+// = 2^-10 of x / mi; that never crosses a half-integer, since x / mi is at
+// least 1 / (2 mi) >= 2^-9 away from one for odd mi (for mi = 256, c22 is
+// exact and a tie rounds up: -128).  So q = round(x / mi) and r = x - q mi.
+// Two mad24 and a shift (|x c22| < 2^28).
+__device__ __forceinline__ int mod_k(int x, int c22, int mi) {
+  const int q = (__mul24(x, c22) + (1 << 21)) >> 22;
+  return __mul24(q, -mi) + x;
+}
 
 // symmetric residue of the FP64 integer x (|x| <= 2^63) modulo mi: the FP64
 // quotient estimate x * inv is within |x / mi| 2^-52 <= 11 of x / mi, so r =
-// x - q mi is an integer with |r| < 12 mi, exact in one fma; mod_small finishes
-__device__ __forceinline__ int sym_residue(double x, int mi, double inv) {
+// x - q mi is an integer with |r| < 12 mi <= 3072, exact in one fma; mod_k
+// finishes
+__device__ __forceinline__ int sym_residue(double x, int mi, double inv, int c22) {
   const double r = fma(-rint(x * inv), (double)mi, x);
-  return mod_small((int)r, (float)mi, 1.0f / (float)mi);
+  return mod_k((int)r, c22, mi);
 }
 
 // residue of a small integer (|x| < 2^22) modulo mi in [-mi/2, mi/2]: the

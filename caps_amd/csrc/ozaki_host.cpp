@@ -129,6 +129,7 @@ bool ozaki_make_consts(int nmod, OzakiConsts &k) {
       if (c > (mods[l] - 1) / 2) c -= mods[l];
       k.c14[l] = c;
     }
+    k.c22[l] = ((1 << 23) / mods[l] + 1) >> 1;  // round(2^22 / m)
     k.inv_m[l] = 1.0 / (double)mods[l];
     M.mul((uint32_t)mods[l]);
   }

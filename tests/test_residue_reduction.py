@@ -86,3 +86,52 @@ def test_reduction_catches_a_wrong_offset():
     ok = (rb >= 0) & (rb <= 255)
     got = (rb[ok].astype(np.int64) ^ 0x80).astype(np.uint8)
     assert not (ok.all() and np.array_equal(got, want_byte(xb - 128, m)))
+
+
+def mod_k(x, m):
+    """mod_k in caps_amd/csrc/ozaki_dev.h: q = (x c22 + 2^21) >> 22, r = x - q m."""
+    c22 = ((1 << 23) // m + 1) >> 1
+    x = np.asarray(x, dtype=np.int64)
+    q = (x * c22 + (1 << 21)) >> 22
+    return x - q * m
+
+
+def sym(x, m):
+    r = np.mod(np.asarray(x, dtype=np.int64) + m // 2, m) - m // 2
+    hi = (m - 1) // 2
+    return np.where(r > hi, r - m, r)
+
+
+def test_mod_k_exhaustive():
+    x = np.arange(-8192, 8193, dtype=np.int64)
+    for m in split.moduli(16):
+        r = mod_k(x, m)
+        assert np.array_equal(np.mod(r - x, m), np.zeros_like(x)), m
+        assert r.min() >= -(m // 2) and r.max() <= m // 2, m
+        # for odd m the result is the symmetric residue; for m = 256 a tie gives -128
+        assert np.array_equal(r, sym(x, m)) or m % 2 == 0, m
+        assert np.array_equal(r & 0xFF, sym(x, m) & 0xFF), m
+
+
+def test_mod_k_fails_outside_its_range():
+    """Not vacuous: far outside |x| <= 8192 the one-step quotient goes wrong."""
+    x = np.arange(1 << 20, (1 << 20) + 200000, dtype=np.int64)
+    m = 255
+    assert not np.array_equal(mod_k(x, m), sym(x, m))
+
+
+def test_sym_residue_on_large_fp64_integers():
+    """sym_residue: one FP64 quotient step (x * inv rounded, rint, x - q m
+    exact in one fma) leaves |r| < 12 m, then mod_k; the INT8 byte equals
+    the symmetric residue of x for FP64 integers up to 2^63."""
+    rng = np.random.default_rng(21)
+    e = rng.integers(0, 63, 20000)
+    xs = np.trunc(np.ldexp(rng.uniform(-1, 1, 20000), e))
+    xs = np.concatenate([xs, [2.0 ** 63, -2.0 ** 63, 2.0 ** 62 + 2.0 ** 10, 0.0, 1.0, -1.0]])
+    for m in split.moduli(16):
+        inv = 1.0 / m
+        q = np.rint(xs * inv)
+        r = np.array([int(x) - int(qq) * m for x, qq in zip(xs, q)], dtype=np.int64)
+        assert np.abs(r).max() < 12 * m
+        want = np.array([((int(x) + m // 2) % m) - m // 2 for x in xs], dtype=np.int64)
+        assert np.array_equal(mod_k(r, m) & 0xFF, want & 0xFF), m
