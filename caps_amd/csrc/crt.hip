@@ -271,8 +271,7 @@ __global__ __launch_bounds__(T) void rew_kernel(const RPtrs *__restrict__ tab, l
   using Op = Ew<OP>;
   const RPtrs &P = tab[blockIdx.y];
   const int l = blockIdx.z;
-  const int mi = kc->m[l];
-  const float inv = 1.0f / (float)mi;
+  const int mi = kc->m[l], c22 = kc->c22[l];  // sums of at most 4 residues: |y| <= 512
   const int8_t *in[Op::NIN];
   int8_t *out[Op::NOUT];
 #pragma unroll
@@ -297,7 +296,7 @@ __global__ __launch_bounds__(T) void rew_kernel(const RPtrs *__restrict__ tab, l
         Op::apply(x, y);
 #pragma unroll
         for (int k = 0; k < Op::NOUT; k++)
-          o[k] |= (unsigned)(mod_i32(y[k], mi, inv) & 0xff) << (8 * b);
+          o[k] |= (unsigned)(mod_k(y[k], c22, mi) & 0xff) << (8 * b);
       }
 #pragma unroll
       for (int k = 0; k < Op::NOUT; k++) reinterpret_cast<unsigned *>(out[k])[i] = o[k];
@@ -309,7 +308,7 @@ __global__ __launch_bounds__(T) void rew_kernel(const RPtrs *__restrict__ tab, l
       for (int k = 0; k < Op::NIN; k++) x[k] = (int)in[k][i];
       Op::apply(x, y);
 #pragma unroll
-      for (int k = 0; k < Op::NOUT; k++) out[k][i] = (int8_t)mod_i32(y[k], mi, inv);
+      for (int k = 0; k < Op::NOUT; k++) out[k][i] = (int8_t)mod_k(y[k], c22, mi);
     }
   }
 }

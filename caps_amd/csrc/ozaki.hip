@@ -649,7 +649,7 @@ __global__ __launch_bounds__(T256) void oz_bgemm_kernel(const RGemm *__restrict_
         int res[4];
 #pragma unroll
         for (int q = 0; q < 4; q++)
-          res[q] = mod_small((int)fold_f32(acc[r][c][q], mf, invf), mf, invf);
+          res[q] = mod_k((int)fold_f32(acc[r][c][q], mf, invf), kc->c22[l], mi);  // |.| <= 1.5 m
         if (CHECK == 0) {
           unsigned v = 0;
 #pragma unroll

@@ -93,14 +93,6 @@ __device__ __forceinline__ int mod_small(int x, float mf, float inv) {
   return (int)fmaf(-rintf(xf * inv), mf, xf);
 }
 
-// residue of an INT32 product sum with |x| < 2^28 (K <= 2^14 products of
-// INT8 residues): x = xh 2^14 + xl with 0 <= xl < 2^14, 2^14 = c14 (mod mi),
-// |xh c14| <= 2^14 * 128, so y = xh c14 + xl < 2^22 goes to mod_small.
-__device__ __forceinline__ int mod_acc(int x, int c14, float mf, float inv) {
-  const int y = __mul24(x >> 14, c14) + (x & 16383);
-  return mod_small(y, mf, inv);
-}
-
 // byte q of a packed INT8 residue word from an INT32 product sum that started
 // at 128: xb = x + 128, |x| <= 2^28 (K <= 2^14 products of INT8 residues).
 //   y = xb + (xb >> 14) (c14 - 2^14) = (xb >> 14) c14 + (xb & 16383) = xb
