@@ -98,3 +98,21 @@ def test_feature_restatement_matches_library():
     assert np.array_equal(got, np.array(want, dtype=np.float32))
     # leaves 512 (256-aligned), 576, 544, 432 (64-, 32-, 16-aligned), 7 (odd)
     assert set(np.unique(got[:, 7]).tolist()) == {1.0, 0.75, 0.625, 0.5, 0.0}
+
+
+def test_planner_shapes():
+    """The seeded shapes of caps_amd.planner_data: n a multiple of 256 in
+    [512, 16384], the deepest layout with leaves >= 512 that divides n, spread
+    in 0..20; the training and held-out seeds give different shape lists."""
+    from caps_amd import planner_data as pd
+    for split in ("train", "test"):
+        for i in range(300):
+            n, r, spread = pd.shape(split, i)
+            assert n % 256 == 0 and 512 <= n <= 16384
+            assert n % (1 << r) == 0 and (n >> r) >= 512
+            assert not (n % (2 << r) == 0 and (n >> (r + 1)) >= 512)
+            assert 0 <= spread <= 20
+    train = [pd.shape("train", i) for i in range(50)]
+    test = [pd.shape("test", i) for i in range(50)]
+    assert train != test
+    assert pd.shape("train", 7) == pd.shape("train", 7)
