@@ -127,6 +127,7 @@ SIGNATURES = {
     "caps_autotune": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.POINTER(Candidate), ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Candidate), _vp]),
+    "caps_plan_choice": (ctypes.c_int, [_D, _O, ctypes.POINTER(Candidate)]),
     "caps_planner_config": (ctypes.c_int, [ctypes.c_double, ctypes.c_int, ctypes.c_double]),
     "caps_tune_save": (ctypes.c_int, [ctypes.c_char_p]),
     "caps_tune_load": (ctypes.c_int, [ctypes.c_char_p]),
@@ -312,8 +313,21 @@ def reset_counters() -> None:
 
 
 def set_available_memory(doubles: int) -> None:
-    """setAvailableMemory (memory.cpp:31-33), in doubles."""
+    """setAvailableMemory (memory.cpp:31-33), in doubles; 0 restores the
+    default budget (half of the free device memory)."""
     _check(load().caps_set_available_memory(int(doubles)))
+
+
+def plan_choice(desc: MatDescriptor, mode: int = MODE_FP64, moduli: int = 0, flags: int = 0,
+                cutoff: int = 0) -> dict:
+    """What a call with pattern None and these options would run: leaf mode,
+    moduli, cutoff and pattern (the automatic choice of multiplyInternal,
+    multiply.cpp:43-66, under the memory budget)."""
+    c = Candidate()
+    o = _opts(mode, moduli, flags, cutoff)
+    _check(load().caps_plan_choice(ctypes.byref(desc), ctypes.byref(o), ctypes.byref(c)))
+    return {"mode": c.mode, "moduli": c.moduli, "cutoff": c.cutoff,
+            "pattern": c.pattern.decode()}
 
 
 def release() -> None:

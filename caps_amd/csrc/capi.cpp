@@ -57,6 +57,9 @@ struct State {
   double timers_ms[CAPS_NUM_TIMERS] = {};
   long long timed[CAPS_NUM_TIMERS] = {};
   long long available_doubles = 1LL << 28;  // memory.cpp:4
+  // set by caps_set_available_memory; unset, the planner takes half of the
+  // free device memory
+  bool budget_set = false;
   // CAPS_MODE_AUTO: error target, allowed modes, expected exponent spread
   double auto_max_err = 1e-14;
   int auto_mask = CAPS_MASK_FP64 | CAPS_MASK_CRT_INT8;
@@ -131,7 +134,8 @@ int check_desc(const caps_desc *d) {
 }
 
 int resolve_pattern(const caps_desc &d, const char *pattern, const caps_opts *o,
-                    std::string &out, int mode, int moduli, long long budget_doubles) {
+                    std::string &out, int mode, int moduli, long long budget_doubles,
+                    bool budget_set) {
   if (pattern) {
     out.clear();
     for (int l = 0; l < d.nrec; l++) {
@@ -148,7 +152,7 @@ int resolve_pattern(const caps_desc &d, const char *pattern, const caps_opts *o,
     out.assign((size_t)d.nrec, 'd');  // one process: DFS at every level (multiply.cpp:43-66)
     return CAPS_OK;
   }
-  out = choose_pattern(d, mode, moduli, budget_doubles);
+  out = choose_pattern(d, mode, moduli, budget_doubles, budget_set);
   return CAPS_OK;
 }
 
@@ -226,8 +230,8 @@ int make_plan_locked(State &s, const PlanKey &key, F build, Plan *&out) {
 
 int get_plan_locked(State &s, const double *A, const double *B, double *C, const caps_desc &d,
                     const std::string &pat, int mode, int moduli, int cutoff, bool fuse,
-                    Plan *&out) {
-  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff, fuse, std::string()};
+                    hipStream_t stream, Plan *&out) {
+  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff, fuse, std::string(), stream};
   for (auto &p : s.plans)
     if (p->key == key) {
       out = p.get();
@@ -268,13 +272,14 @@ int auto_choose(State &s, const caps_desc &d, const char *pattern, int &mode, in
                 int &cutoff, std::string &pat) {
   TuneEntry e;
   if (tune_lookup(d, s.auto_mask, s.auto_max_err, e)) {
-    mode = e.c.mode;
-    moduli = e.c.moduli ? e.c.moduli : OZ_DEFAULT_MODULI;
+    // a table entry goes through the same checks as a caller's options
+    caps_opts te{e.c.mode, e.c.moduli, 0, e.c.cutoff};
+    int rc = resolve_mode(d, &te, mode, moduli);
+    if (rc) return rc;
     cutoff = e.c.cutoff;
-    if (!pattern && (int)strlen(e.c.pattern) >= d.nrec) {
-      pat.assign(e.c.pattern, (size_t)d.nrec);
-      return CAPS_OK;
-    }
+    if (!pattern && (int)strlen(e.c.pattern) >= d.nrec)
+      return resolve_pattern(d, e.c.pattern, &te, pat, mode, moduli, s.available_doubles,
+                             s.budget_set);
   } else if (mlp_planner_choose(d, s.auto_mask, s.auto_max_err, s.auto_spread, mode, moduli,
                                 cutoff, pat)) {
     if (!pattern) return CAPS_OK;
@@ -285,7 +290,7 @@ int auto_choose(State &s, const caps_desc &d, const char *pattern, int &mode, in
   }
   if (mode == CAPS_MODE_FP64) moduli = 0;
   caps_opts o{mode, moduli, 0, cutoff};
-  return resolve_pattern(d, pattern, &o, pat, mode, moduli, s.available_doubles);
+  return resolve_pattern(d, pattern, &o, pat, mode, moduli, s.available_doubles, s.budget_set);
 }
 
 int multiply_device_locked(State &s, const double *dA, const double *dB, double *dC,
@@ -301,14 +306,16 @@ int multiply_device_locked(State &s, const double *dA, const double *dB, double 
     if (rc) return rc;
     if (pattern) {  // the caller's pattern with the chosen leaf mode
       caps_opts o{mode, moduli, 0, cutoff};
-      rc = resolve_pattern(*desc, pattern, &o, pat, mode, moduli, s.available_doubles);
+      rc = resolve_pattern(*desc, pattern, &o, pat, mode, moduli, s.available_doubles,
+                           s.budget_set);
       if (rc) return rc;
     }
   } else {
     rc = resolve_mode(*desc, opts, mode, moduli);
     if (rc) return rc;
     cutoff = opts ? opts->cutoff : 0;
-    rc = resolve_pattern(*desc, pattern, opts, pat, mode, moduli, s.available_doubles);
+    rc = resolve_pattern(*desc, pattern, opts, pat, mode, moduli, s.available_doubles,
+                         s.budget_set);
     if (rc) return rc;
   }
   rc = check_cutoff(*desc, mode, cutoff);
@@ -317,7 +324,7 @@ int multiply_device_locked(State &s, const double *dA, const double *dB, double 
     return fail(CAPS_EUNSUPPORTED, "residue-domain mode supports nrec <= 7 (got %d)", desc->nrec);
   Plan *plan = nullptr;
   const bool fuse = !(opts && (opts->flags & CAPS_FLAG_NO_FUSION));
-  rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, cutoff, fuse, plan);
+  rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, cutoff, fuse, stream, plan);
   if (rc) return rc;
   const bool timed = opts && (opts->flags & CAPS_FLAG_TIMERS);
   const bool use_graph = !timed && !(opts && (opts->flags & CAPS_FLAG_NO_GRAPH)) &&
@@ -516,10 +523,11 @@ int caps_reset_counters(void) {
 }
 
 int caps_set_available_memory(long long doubles) {
-  if (doubles <= 0) return fail(CAPS_EINVAL, "available memory must be positive");
+  if (doubles < 0) return fail(CAPS_EINVAL, "available memory must not be negative");
   State &s = st();
   std::lock_guard<std::mutex> g(s.mu);
-  s.available_doubles = doubles;
+  s.budget_set = doubles > 0;
+  s.available_doubles = doubles > 0 ? doubles : 1LL << 28;
   return CAPS_OK;
 }
 
@@ -551,6 +559,11 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
     }
   }
   std::vector<BatchItem> items((size_t)count);
+  // items of one group are batched together: each NULL-pattern item gets an
+  // equal share of the budget
+  const long long share =
+      (long long)(planner_budget_bytes(s.available_doubles, s.budget_set) / sizeof(double)) /
+      count;
   std::string key;
   key.reserve((size_t)count * 64);
   for (int i = 0; i < count; i++) {
@@ -569,7 +582,7 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
     items[i].cutoff = cutoff;
     caps_opts o{mode, moduli, 0, cutoff};
     rc = resolve_pattern(descs[i], patterns ? patterns[i] : nullptr, &o, items[i].pattern, mode,
-                         moduli, s.available_doubles / count);
+                         moduli, share > 0 ? share : 1, true);
     if (rc) return rc;
     char buf[176];
     snprintf(buf, sizeof(buf), "%p %p %p %d %d %d %d %s;", (const void *)dA[i],
@@ -577,7 +590,8 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
              items[i].pattern.c_str());
     key += buf;
   }
-  PlanKey pk{nullptr, nullptr, nullptr, descs[0], std::string(), mode, moduli, 0, true, key};
+  PlanKey pk{nullptr, nullptr, nullptr, descs[0], std::string(), mode, moduli, 0, true, key,
+             stream};
   Plan *plan = nullptr;
   for (auto &p : s.plans)
     if (p->key == pk) {
@@ -661,7 +675,8 @@ int caps_measure(const double *dA, const double *dB, double *dC, const caps_desc
     pat = cand->pattern;
   } else {
     int mode = cand->mode, moduli = cand->moduli ? cand->moduli : OZ_DEFAULT_MODULI;
-    rc = resolve_pattern(*desc, nullptr, &o, pat, mode, moduli, s.available_doubles);
+    rc = resolve_pattern(*desc, nullptr, &o, pat, mode, moduli, s.available_doubles,
+                         s.budget_set);
     if (rc) return rc;
     snprintf(cand->pattern, sizeof(cand->pattern), "%s", pat.c_str());
   }
@@ -692,6 +707,34 @@ int caps_measure(const double *dA, const double *dB, double *dC, const caps_desc
     HIP_TRY(normwise_error(dC, dRef, (long long)desc->lda * desc->lda, &err, hs));
     cand->err = err;
   }
+  return CAPS_OK;
+}
+
+int caps_plan_choice(const caps_desc *desc, const caps_opts *opts, caps_candidate *out) {
+  if (!out) return fail(CAPS_EINVAL, "caps_plan_choice: NULL output");
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = check_desc(desc);
+  if (rc) return rc;
+  int mode, moduli, cutoff;
+  std::string pat;
+  if (opts && opts->mode == CAPS_MODE_AUTO) {
+    rc = auto_choose(s, *desc, nullptr, mode, moduli, cutoff, pat);
+  } else {
+    rc = resolve_mode(*desc, opts, mode, moduli);
+    cutoff = opts ? opts->cutoff : 0;
+    if (!rc) rc = resolve_pattern(*desc, nullptr, opts, pat, mode, moduli, s.available_doubles,
+                                  s.budget_set);
+  }
+  if (!rc) rc = check_cutoff(*desc, mode, cutoff);
+  if (rc) return rc;
+  memset(out, 0, sizeof(*out));
+  out->mode = mode;
+  out->moduli = moduli;
+  out->cutoff = cutoff;
+  snprintf(out->pattern, sizeof(out->pattern), "%s", pat.c_str());
+  out->ms = -1;
+  out->err = -1;
   return CAPS_OK;
 }
 

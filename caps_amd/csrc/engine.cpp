@@ -34,7 +34,7 @@ namespace caps {
 bool PlanKey::operator==(const PlanKey &o) const {
   return A == o.A && B == o.B && C == o.C && memcmp(&desc, &o.desc, sizeof(desc)) == 0 &&
          pattern == o.pattern && mode == o.mode && moduli == o.moduli && cutoff == o.cutoff &&
-         fuse == o.fuse && batch == o.batch;
+         fuse == o.fuse && batch == o.batch && stream == o.stream;
 }
 
 namespace {
@@ -83,7 +83,10 @@ struct Builder {
   void ensure_consts() {
     if (plan.oz_moduli == 0) {
       OzakiConsts k;
-      ozaki_make_consts(moduli, k);
+      if (!ozaki_make_consts(moduli, k)) {  // modulus count outside 2..16
+        broken = true;
+        return;
+      }
       plan.oz_const = add_table(&k, sizeof(k));
       plan.oz_moduli = moduli;
     }

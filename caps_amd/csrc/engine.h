@@ -59,6 +59,9 @@ struct Step {
   size_t aux;     // byte offset of an auxiliary workspace region
 };
 
+// A plan owns one device workspace, so it belongs to one stream: calls with
+// the same buffers on different streams get separate plans and never share
+// a workspace.
 struct PlanKey {
   const void *A, *B, *C;
   caps_desc desc;
@@ -66,6 +69,7 @@ struct PlanKey {
   int mode, moduli, cutoff;
   bool fuse;
   std::string batch;  // batched plans: every item's pointers, descriptor and pattern
+  const void *stream;
   bool operator==(const PlanKey &o) const;
 };
 

@@ -6,10 +6,15 @@
 
 namespace caps {
 
+// Workspace budget in bytes: the caller's (budget_set, capped at the free
+// device memory) or, by default, half of the free device memory.
+size_t planner_budget_bytes(long long budget_doubles, bool budget_set);
+
 // One char per level: 'b' (batched) at every level whose workspace still fits
 // the budget, 'd' below.  On one device a 'b' level only costs memory
 // ((7/4)^levels growth), never communication, so batching is preferred.
-std::string choose_pattern(const caps_desc &d, int mode, int moduli, long long budget_doubles);
+std::string choose_pattern(const caps_desc &d, int mode, int moduli, long long budget_doubles,
+                           bool budget_set);
 
 // Leaf arithmetic when the caller asks for CAPS_MODE_AUTO.
 int choose_mode(const caps_desc &d);

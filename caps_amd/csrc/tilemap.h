@@ -40,8 +40,10 @@ __device__ __forceinline__ void tile_coords(const TileMap &tm, int t, int &prob,
 }
 
 // Block b of a persistent grid of G blocks (G a multiple of NXCD) runs on XCD
-// b % NXCD; give each XCD a contiguous range of the round's G tile ids.
-__device__ __forceinline__ int xcd_tile(int b, int G) {
+// b % NXCD; give each XCD a contiguous range of the round's G tile ids.  A
+// permutation of [0, G) exactly when NXCD divides G, which persistent_grid
+// guarantees.
+__host__ __device__ __forceinline__ int xcd_tile(int b, int G) {
   return (b % NXCD) * (G / NXCD) + b / NXCD;
 }
 
@@ -53,9 +55,14 @@ __device__ __forceinline__ long long quad_leaf(int bi, int bj, int d) {
   return L;
 }
 
+// Grid size of a persistent launch: one round of blocks_per_cu blocks per CU,
+// at most enough for `total` tiles, always a positive multiple of NXCD (so
+// xcd_tile is a permutation; a CU count that is not a multiple of NXCD is
+// rounded down).
 inline int persistent_grid(int blocks_per_cu, int cus, int total) {
-  int G = cus * blocks_per_cu;
-  if (G > total) G = ((total + NXCD - 1) / NXCD) * NXCD;
+  int G = (cus * blocks_per_cu / NXCD) * NXCD;
+  const int need = ((total + NXCD - 1) / NXCD) * NXCD;
+  if (G > need) G = need;
   if (G < NXCD) G = NXCD;
   return G;
 }

@@ -124,25 +124,52 @@ int tune_save(const char *path) {
   return CAPS_OK;
 }
 
+// one line of a tuning file: every field in range, or false
+bool valid_entry(const TuneEntry &e, const char *pat) {
+  if (e.lda <= 0 || e.nrec < 0 || e.nrec > 24 || e.bs <= 0 || !(e.max_err > 0)) return false;
+  if (e.c.mode != CAPS_MODE_FP64 && e.c.mode != CAPS_MODE_OZAKI_INT8 &&
+      e.c.mode != CAPS_MODE_CRT_INT8 && e.c.mode != CAPS_MODE_CRT_BF16)
+    return false;
+  if (e.c.mode == CAPS_MODE_FP64 ? e.c.moduli != 0
+                                 : (e.c.moduli != 0 && (e.c.moduli < 2 || e.c.moduli > 16)))
+    return false;
+  if (e.c.cutoff < 0 || e.c.cutoff > e.nrec) return false;
+  if (strcmp(pat, "-") == 0) return true;
+  if ((int)strlen(pat) != e.nrec) return false;
+  for (const char *c = pat; *c; c++)
+    if (!strchr("bBdDhH", *c)) return false;
+  return true;
+}
+
+// All-or-nothing: a file with any malformed entry loads nothing.
 int tune_load(const char *path) {
   FILE *f = fopen(path, "r");
   if (!f) return set_error(CAPS_EINVAL, "caps_tune_load: cannot open file");
   char line[512];
-  int n = 0;
+  std::vector<TuneEntry> got;
+  int lineno = 0;
   while (fgets(line, sizeof(line), f)) {
-    if (line[0] == '#') continue;
+    lineno++;
+    const char *p = line;
+    while (*p == ' ' || *p == '\t') p++;
+    if (*p == '#' || *p == '\n' || *p == '\0') continue;
     TuneEntry e{};
     char pat[64] = {0};
-    if (sscanf(line, "%d %d %d %d %lg %d %d %d %63s %lg %lg", &e.lda, &e.nrec, &e.bs,
+    if (sscanf(p, "%d %d %d %d %lg %d %d %d %63s %lg %lg", &e.lda, &e.nrec, &e.bs,
                &e.mode_mask, &e.max_err, &e.c.mode, &e.c.moduli, &e.c.cutoff, pat, &e.c.ms,
-               &e.c.err) == 11) {
-      snprintf(e.c.pattern, sizeof(e.c.pattern), "%s", strcmp(pat, "-") ? pat : "");
-      tune_store(e);
-      n++;
+               &e.c.err) != 11 ||
+        !valid_entry(e, pat)) {
+      fclose(f);
+      char msg[96];
+      snprintf(msg, sizeof(msg), "caps_tune_load: malformed entry on line %d", lineno);
+      return set_error(CAPS_EINVAL, msg);
     }
+    snprintf(e.c.pattern, sizeof(e.c.pattern), "%s", strcmp(pat, "-") ? pat : "");
+    got.push_back(e);
   }
   fclose(f);
-  return n >= 0 ? CAPS_OK : CAPS_EINVAL;
+  for (const TuneEntry &e : got) tune_store(e);
+  return CAPS_OK;
 }
 
 }  // namespace caps

@@ -89,7 +89,9 @@ typedef struct caps_opts {
 } caps_opts;
 
 /* Re-use a captured hipGraph for repeated calls with the same pointers,
- * descriptor, pattern and options (default on). */
+ * descriptor, pattern, options and stream (default on).  A cached plan owns
+ * one device workspace and belongs to one stream: calls on different streams
+ * never share a workspace, and calls on one stream run in stream order. */
 #define CAPS_FLAG_NO_GRAPH 1
 /* Force the reference's automatic rule when pattern == NULL (DFS at every
  * level on one process, multiply.cpp:43-66) instead of the planner's choice. */
@@ -184,7 +186,13 @@ int caps_get_counters(caps_counters *out);
 int caps_reset_counters(void);
 
 /* Memory budget in doubles for the automatic BFS/DFS choice
- * (setAvailableMemory, memory.cpp:31-33; default 2^28 doubles). */
+ * (setAvailableMemory, memory.cpp:31-33).  With a NULL pattern a level is
+ * batched ('b') only while the plan's device workspace fits the budget,
+ * greedily from the top level, else recursive ('d'), as enoughMemory
+ * (memory.cpp:22-29) gates the reference's BFS steps.  doubles > 0 sets the
+ * budget (capped at the free device memory); 0 restores the default, half of
+ * the free device memory (2^28 doubles, memory.cpp:4, when no device
+ * answers). */
 int caps_set_available_memory(long long doubles);
 
 /* Release cached plans, graphs and workspace. */
@@ -230,6 +238,13 @@ int caps_measure(const double *dA, const double *dB, double *dC, const caps_desc
 int caps_autotune(const double *dA, const double *dB, double *dC, const caps_desc *desc,
                   int mode_mask, double max_err, int reps, caps_candidate *out, int max_out,
                   int *n_out, caps_candidate *best, void *stream);
+/* The leaf mode, moduli, cutoff and pattern that a call with pattern NULL
+ * and these options would run (opts NULL: FP64 with the memory-budget
+ * pattern rule above; CAPS_MODE_AUTO: the tuning table, then the learned
+ * model, then the default), without multiplying.  Needs no device unless the
+ * learned model is consulted.  multiplyInternal's automatic choice,
+ * multiply.cpp:43-66. */
+int caps_plan_choice(const caps_desc *desc, const caps_opts *opts, caps_candidate *out);
 /* The error target and allowed modes that CAPS_MODE_AUTO calls look up. */
 int caps_planner_config(double max_err, int mode_mask, double exponent_spread);
 int caps_tune_save(const char *path);
