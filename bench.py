@@ -11,14 +11,21 @@ Workload (one "step" = one multiply of device-resident CAPS-layout inputs):
   reconstruction) is the reported value; the plain FP64 run of the same
   multiply and the leaf-only split (FP64 additions) are reported beside it.
   After timing:
-    * normwise error max|C - C*| / max|C*| on sampled rows against the
+    * normwise error max|C - C*| / max|C*| over all of C against the device
+      residue product with 16 moduli, and on 16 sampled rows against the
       compensated (Dot2) classical product from the CPU oracle,
     * an integer pair in [-64, 64] through the split path checked exactly
       (Freivalds' test over all of C plus sampled rows, exact in FP64).
-The roofline entry is for the leaf GEMM launch (INT8 MFMA), timed live with
-HIP events on the launch stream through the library's phase timers.
+Beside it (config): plain FP64 at n = 7168, r = 3, 'bdd'; the uneven batch
+of 512 problems (n in [128, 4096], plans from the learned planner) in FP64
+and split precision.  The roofline entry is for the leaf GEMM launch (INT8
+MFMA), timed live with HIP events on the launch stream through the
+library's phase timers (20 eager launches by default).  cpu_baseline is the
+reference's own randombenchmark built from its sources (oracle/ref.mk) when
+present, else the oracle's restatement.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--n 8192]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+                    [--workload split|batch]
 """
 from __future__ import annotations
 
@@ -115,10 +122,34 @@ def exact_integer_check(A_cm, B_cm, C_cm, rows, seed=11):
     return ok
 
 
-def cpu_baseline(oracle, n=4096, r=2, threads=None):
-    """The oracle's Winograd restatement (multiply.cpp:409-454 order, blocked
-    leaf dgemm) on the host cores: one multiply at n=4096, nrec=2, 'dd'."""
-    threads = threads or max(1, min(16, os.cpu_count() or 1))
+def cpu_threads():
+    """Threads for the CPU leg: the host cores this process may use, capped by
+    OMP_NUM_THREADS when set (a shared GPU host gives each GPU a share)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(avail, int(omp))) if omp and omp.isdigit() else avail
+
+
+def cpu_baseline(oracle, n=8192, r=2):
+    """The reference's own randombenchmark (built from its sources into
+    oracle/_ref by oracle/ref.mk, MKL dgemm_) at the metric's size, one MPI
+    process, all DFS levels (the reference's one-process path,
+    multiply.cpp:43-66); else the oracle's restatement at n=4096."""
+    threads = cpu_threads()
+    host = {"host_cpu_count": os.cpu_count(), "threads_used": threads,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    ref = oracle.run_reference_benchmark(n, r, "d" * r, threads)
+    if ref is not None:
+        return {"value": round(ref["effective_gflops"], 3), "unit": "GFLOP/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"reference randombenchmark -s {n} -r {r} -p {'d' * r} (one process, "
+                          f"NUM_THREADS=16 OpenMP additions, MKL dgemm_ with {threads} threads), "
+                          f"{ref['time_s']:.2f} s; effective Gflop/s = 2n^3/t (counters.cpp:69)",
+                **host}
+    n = 4096
     g = oracle.Drand48()
     a = g.uniform(n * n)
     b = g.uniform(n * n)
@@ -128,112 +159,109 @@ def cpu_baseline(oracle, n=4096, r=2, threads=None):
     return {"value": round(2.0 * n ** 3 / t / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
             "kind": "port",
             "sample": f"one oracle multiply n={n} nrec={r} pattern 'dd' bs=1, uniform [-1,1) "
-                      f"from drand48, {threads} OpenMP threads, {t:.2f} s"}
+                      f"from drand48, {threads} OpenMP threads, {t:.2f} s", **host}
 
 
-def batch_sizes(count=512, seed=2024):
-    """n = 64 round(exp(U[ln 128, ln 4096]) / 64), clipped to [128, 4096]."""
-    rng = np.random.default_rng(seed)
-    n = 64 * np.round(np.exp(rng.uniform(np.log(128), np.log(4096), count)) / 64)
-    return [int(x) for x in np.clip(n, 128, 4096)]
+def run_uneven_batch(torch, caps_amd, sizes, plans, mode, steps, warmup, check=True,
+                     compare_single=False):
+    """512 independent problems (entries uniform [-1, 1)) in one batched plan,
+    replayed as one graph: ms per batch, effective GFLOP/s (sum of 2n^3 / t)
+    and the worst normwise error against the 16-modulus reference product."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(77)
+    As, Bs, Cs, descs, pats, cuts = [], [], [], [], [], []
+    for n in sizes:
+        p = plans[n]
+        As.append(torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1)
+        Bs.append(torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1)
+        Cs.append(torch.empty_like(As[-1]))
+        descs.append(caps_amd.MatDescriptor.one_device(n, p.nrec, 1))
+        pats.append(p.pattern)
+        cuts.append(p.cutoff)
+    K = next(iter(plans.values())).moduli
 
+    def run_batch():
+        caps_amd.multiply_batch_device(As, Bs, Cs, descs, pats, mode, K, cutoffs=cuts)
 
-def tune_one_size(torch, caps_amd, n, mode, moduli, reps=2):
-    """Fastest (nrec, pattern, cutoff) for one size: layout depths with
-    leaves >= 64, every candidate the library enumerates for the mode at the
-    given modulus count, timed on the device (caps_measure)."""
-    mask = {caps_amd.MODE_FP64: caps_amd.MASK_FP64,
-            caps_amd.MODE_CRT_INT8: caps_amd.MASK_CRT_INT8}[mode]
-    best = None
-    for r in range(0, 7):
-        if n % (1 << r) or (n >> r) < 64:
-            continue
-        desc = caps_amd.MatDescriptor.one_device(n, r, 1)
-        A = torch.rand(n * n, device="cuda", dtype=torch.float64) * 2 - 1
-        B = torch.rand(n * n, device="cuda", dtype=torch.float64) * 2 - 1
-        C = torch.empty_like(A)
-        for c in caps_amd.candidates(desc, mask):
-            if mode != caps_amd.MODE_FP64 and c.moduli != moduli:
-                continue
-            caps_amd.measure(A, B, C, desc, c, None, reps)
-            if best is None or c.ms < best[3]:
-                best = (r, c.pattern.decode(), c.cutoff, c.ms)
-    return best
-
-
-def batch_workload(args, torch, caps_amd):
-    """512 independent square problems, n log-uniform in [128, 4096] (multiples
-    of 64), entries uniform [-1, 1); per-size depth / pattern / cutoff from
-    timing every candidate; all 512 in one batched plan replayed as one graph."""
-    sizes = batch_sizes()
-    distinct = sorted(set(sizes))
     eff = sum(2.0 * n ** 3 for n in sizes)
-    K = args.moduli
-    out = {}
-    for mode, name in ((caps_amd.MODE_FP64, "fp64"), (caps_amd.MODE_CRT_INT8, "split")):
-        log(f"[bench] batch: tuning {len(distinct)} sizes ({name})")
-        t0 = time.perf_counter()
-        choice = {}
-        for n in distinct:
-            choice[n] = tune_one_size(torch, caps_amd, n, mode, K)
-            log(f"[bench] batch: n={n} -> nrec={choice[n][0]} pattern={choice[n][1]} "
-                f"cutoff={choice[n][2]} {choice[n][3]:.3f} ms")
-        tune_s = time.perf_counter() - t0
-        caps_amd.release()
-        g = torch.Generator(device="cuda")
-        g.manual_seed(77)
-        As, Bs, Cs, descs, pats, cuts = [], [], [], [], [], []
-        for n in sizes:
-            r, pat, cut, _ = choice[n]
-            As.append(torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1)
-            Bs.append(torch.rand(n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1)
-            Cs.append(torch.empty_like(As[-1]))
-            descs.append(caps_amd.MatDescriptor.one_device(n, r, 1))
-            pats.append(pat)
-            cuts.append(cut)
-        moduli = 0 if mode == caps_amd.MODE_FP64 else K
-
-        def run_batch():
-            caps_amd.multiply_batch_device(As, Bs, Cs, descs, pats, mode, moduli, cutoffs=cuts)
-
+    t = time_steps(torch, run_batch, steps, warmup)
+    out = {"ms": round(t * 1e3, 3), "gflops": round(eff / t / 1e9, 2), "moduli": K,
+           "depths": sorted(set(p.nrec for p in plans.values())),
+           "distinct_sizes": len(plans)}
+    if compare_single:
         def run_each():
-            for a, b, c, d, p, k in zip(As, Bs, Cs, descs, pats, cuts):
-                caps_amd.multiply_device(a, b, c, d, p, mode=mode, moduli=moduli, cutoff=k)
-
-        log(f"[bench] batch: timing ({name})")
-        t_batch = time_steps(torch, run_batch, args.steps, args.warmup)
-        t_each = time_steps(torch, run_each, max(2, args.steps // 2), 1)
+            for a, b, c, d, pt, k in zip(As, Bs, Cs, descs, pats, cuts):
+                caps_amd.multiply_device(a, b, c, d, pt, mode=mode, moduli=K, cutoff=k)
+        t1 = time_steps(torch, run_each, max(2, steps // 2), 1)
+        out["ms_one_call_per_problem"] = round(t1 * 1e3, 3)
+        out["gflops_one_call_per_problem"] = round(eff / t1 / 1e9, 2)
         run_batch()
+    if check:
         torch.cuda.synchronize()
-        log(f"[bench] batch {name}: {t_batch * 1e3:.2f} ms (one call per problem: "
-            f"{t_each * 1e3:.2f} ms); errors vs the 16-modulus reference product")
         worst = 0.0
-        for a, b, c, d in zip(As, Bs, Cs, descs):
+        for i, (a, b, c, d) in enumerate(zip(As, Bs, Cs, descs)):
             ref = torch.empty_like(c)
             caps_amd.reference_product(a, b, ref, d)
             worst = max(worst, caps_amd.normwise_error(c, ref, d))
-        out[name] = {"ms": t_batch * 1e3, "gflops": eff / t_batch / 1e9,
-                     "ms_one_call_per_problem": t_each * 1e3,
-                     "gflops_one_call_per_problem": eff / t_each / 1e9,
-                     "max_error": worst, "tuning_s": tune_s,
-                     "choices": {str(n): list(choice[n][:3]) for n in distinct}}
-        del As, Bs, Cs
+            del ref
+            if i % 64 == 63:
+                caps_amd.release()
+        out["max_error"] = worst
+        out["error_metric"] = ("max over problems of max|C-C*|/max|C*|, C* = 16-modulus "
+                               "residue product (caps_reference_product)")
+    del As, Bs, Cs
+    caps_amd.release()
+    torch.cuda.empty_cache()
+    return out
+
+
+def uneven_batch_section(args, torch, caps_amd, measure=False, compare_single=False):
+    """Cfg4: 512 sizes n = 64 round(exp(U[ln 128, ln 4096]) / 64) (seed 2024);
+    per-size depth, pattern and cutoff proposed by the shipped learned planner
+    (measure=True: the proposal at every depth timed on the device, fastest
+    kept); FP64 and residue-domain INT8 both run."""
+    from caps_amd import batch_plan, planner_train
+    model, _ = planner_train.load_model(os.path.join(ROOT, "caps_amd", "planner_model.npz"))
+    sizes = batch_plan.uneven_sizes()
+    res = {"workload": "512 independent multiplies, n = 64 round(exp(U[ln128, ln4096])/64), "
+                       "seed 2024, uniform [-1, 1); one batched plan replayed as one graph",
+           "sum_2n3": sum(2.0 * n ** 3 for n in sizes),
+           "plans": "learned planner" + (" + device timing per depth" if measure else "")}
+    for mode, name, err in ((caps_amd.MODE_FP64, "fp64", 1e-12),
+                            (caps_amd.MODE_CRT_INT8, "split", 1e-14)):
+        t0 = time.perf_counter()
+        plans = batch_plan.choose_plans(sizes, mode, max_err=err, spread=0.0,
+                                        measure_with=torch if measure else None)
         caps_amd.release()
-        torch.cuda.empty_cache()
+        log(f"[bench] batch {name}: plans for {len(plans)} sizes in "
+            f"{time.perf_counter() - t0:.1f} s")
+        r = run_uneven_batch(torch, caps_amd, sizes, plans, mode, args.steps, args.warmup,
+                             check=not args.no_check, compare_single=compare_single)
+        r["planning_s"] = round(time.perf_counter() - t0, 2)
+        log(f"[bench] batch {name}: {r['ms']:.2f} ms, {r['gflops']:.0f} GFLOP/s, "
+            f"max error {r.get('max_error', float('nan')):.3e}")
+        res[name] = r
+    caps_amd.MLP.clear_planner()
+    caps_amd.planner_config(1e-14, caps_amd.MASK_FP64 | caps_amd.MASK_CRT_INT8, 10.0)
+    del model
+    return res
+
+
+def batch_workload(args, torch, caps_amd):
+    """The uneven batch as its own bench line (--workload batch): plans
+    confirmed on the device and compared with one call per problem."""
+    res = uneven_batch_section(args, torch, caps_amd, measure=True, compare_single=True)
     result = {
         "metric": "effective GFLOP/s (sum of 2n^3 / t), 512 uneven problems n in [128, 4096]",
-        "value": round(out["split"]["gflops"], 2),
+        "value": res["split"]["gflops"],
         "unit": "GFLOP/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(out["split"]["ms"], 4),
+        "ms_per_step": res["split"]["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 in/out, int8 MFMA leaf products",
         "data": "synthetic: 512 sizes n = 64 round(exp(U[ln128, ln4096])/64), seed 2024; "
                 "entries uniform [-1, 1)",
-        "config": {"workload": "512 independent multiplies in one batched plan / graph",
-                   "sum_2n3": eff, "moduli": K,
-                   "fp64": {k: v for k, v in out["fp64"].items()},
-                   "split": {k: v for k, v in out["split"].items()}},
+        "config": res,
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -286,6 +314,10 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-fp64-bdd", action="store_true",
                     help="skip the plain FP64 n=7168 r=3 'bdd' measurement")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 512-problem uneven batch (config.batch)")
+    ap.add_argument("--gemm-reps", type=int, default=20,
+                    help="eager launches timed with HIP events for the roofline GEMM")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -350,14 +382,14 @@ def main():
 
     # roofline of the dominant kernel: the leaf GEMM launch (all 49 x K INT8 GEMMs)
     per_launch, per_call = leaf_kernel_ms(caps_amd, torch, dA, dB, dC_split, desc, pat,
-                                          SPLIT, K)
+                                          SPLIT, K, reps=args.gemm_reps)
     m = n >> r
     leaves = 7 ** r
     int_ops = K * 2.0 * leaves * m ** 3
     gemm_ms = per_launch["mul"]
     achieved = int_ops / (gemm_ms * 1e-3) / 1e12
     plain_launch, plain_call = leaf_kernel_ms(caps_amd, torch, dA, dB, dC_plain, desc, pat,
-                                              caps_amd.MODE_FP64, 0)
+                                              caps_amd.MODE_FP64, 0, reps=args.gemm_reps)
     f64_ach = 2.0 * leaves * m ** 3 / (plain_launch["mul"] * 1e-3) / 1e12
     log(f"[bench] split phases per call (ms): {per_call}")
     log(f"[bench] plain phases per call (ms): {plain_call}")
@@ -400,6 +432,15 @@ def main():
     }
 
     if not args.no_check:
+        log("[bench] accuracy: all of C against the 16-modulus reference product")
+        dRef = torch.empty_like(dA)
+        caps_amd.reference_product(dA, dB, dRef, desc)
+        full = {name: caps_amd.normwise_error(c, dRef, desc)
+                for name, c in (("split", dC_split), ("plain", dC_plain), ("leaf_split", dC_leaf),
+                                ("bf16_variant", dC_bf16))}
+        del dRef
+        caps_amd.release()
+        log(f"[bench] full-C errors {full}")
         log("[bench] accuracy: copying operands to host")
         A_cm = oracle.from_caps(dA.cpu().numpy(), n, r, 1)
         B_cm = oracle.from_caps(dB.cpu().numpy(), n, r, 1)
@@ -430,14 +471,25 @@ def main():
         result["config"]["bf16_variant_bitwise_equal_to_int8"] = bf16_same
         result["config"]["error_metric"] = ("max|C-C*|/max|C*| over 16 sampled full rows, "
                                             "C* = Dot2 compensated classical product")
+        result["config"]["error_full_c"] = full
+        result["config"]["error_full_c_metric"] = (
+            "max|C-C*|/max|C*| over all n^2 entries, C* = residue-domain product with 16 "
+            "moduli (operands truncated 63 bits below their row / column norms)")
         result["config"]["integer_pair_bit_exact"] = exact
-        result["config"]["meets_error_bar"] = err_split <= 1e-14
+        result["config"]["meets_error_bar"] = err_split <= 1e-14 and full["split"] <= 1e-14
 
+    del dA, dB, dC_split, dC_plain, dC_leaf, dC_bf16
+    caps_amd.release()
+    torch.cuda.empty_cache()
     if not args.no_fp64_bdd:
         result["config"]["plain_fp64_n7168_bdd"] = fp64_bdd_workload(args, torch, caps_amd, oracle)
 
+    if not args.no_batch:
+        log("[bench] uneven batch: 512 problems, plans from the learned planner")
+        result["config"]["batch"] = uneven_batch_section(args, torch, caps_amd)
+
     if not args.no_cpu:
-        log("[bench] cpu baseline (oracle restatement)")
+        log("[bench] cpu baseline (reference randombenchmark, else the oracle restatement)")
         result["cpu_baseline"] = cpu_baseline(oracle)
     else:
         result["cpu_baseline"] = None

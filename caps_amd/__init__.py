@@ -327,7 +327,7 @@ def plan_choice(desc: MatDescriptor, mode: int = MODE_FP64, moduli: int = 0, fla
     o = _opts(mode, moduli, flags, cutoff)
     _check(load().caps_plan_choice(ctypes.byref(desc), ctypes.byref(o), ctypes.byref(c)))
     return {"mode": c.mode, "moduli": c.moduli, "cutoff": c.cutoff,
-            "pattern": c.pattern.decode()}
+            "pattern": c.pattern.decode(), "predicted_ms": c.ms, "predicted_err": c.err}
 
 
 def release() -> None:

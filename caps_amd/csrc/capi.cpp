@@ -269,7 +269,7 @@ int capture_graph(Plan &p) {
 // CAPS_MODE_AUTO: the tuning table, then the learned model, then a default
 // (residue-domain split when allowed: it meets a 1e-14 bar; else FP64)
 int auto_choose(State &s, const caps_desc &d, const char *pattern, int &mode, int &moduli,
-                int &cutoff, std::string &pat) {
+                int &cutoff, std::string &pat, double *pred = nullptr) {
   TuneEntry e;
   if (tune_lookup(d, s.auto_mask, s.auto_max_err, e)) {
     // a table entry goes through the same checks as a caller's options
@@ -281,7 +281,7 @@ int auto_choose(State &s, const caps_desc &d, const char *pattern, int &mode, in
       return resolve_pattern(d, e.c.pattern, &te, pat, mode, moduli, s.available_doubles,
                              s.budget_set);
   } else if (mlp_planner_choose(d, s.auto_mask, s.auto_max_err, s.auto_spread, mode, moduli,
-                                cutoff, pat)) {
+                                cutoff, pat, pred)) {
     if (!pattern) return CAPS_OK;
   } else {
     mode = (s.auto_mask & CAPS_MASK_CRT_INT8) ? CAPS_MODE_CRT_INT8 : CAPS_MODE_FP64;
@@ -718,8 +718,9 @@ int caps_plan_choice(const caps_desc *desc, const caps_opts *opts, caps_candidat
   if (rc) return rc;
   int mode, moduli, cutoff;
   std::string pat;
+  double pred[2] = {-1.0, -1.0};
   if (opts && opts->mode == CAPS_MODE_AUTO) {
-    rc = auto_choose(s, *desc, nullptr, mode, moduli, cutoff, pat);
+    rc = auto_choose(s, *desc, nullptr, mode, moduli, cutoff, pat, pred);
   } else {
     rc = resolve_mode(*desc, opts, mode, moduli);
     cutoff = opts ? opts->cutoff : 0;
@@ -733,8 +734,8 @@ int caps_plan_choice(const caps_desc *desc, const caps_opts *opts, caps_candidat
   out->moduli = moduli;
   out->cutoff = cutoff;
   snprintf(out->pattern, sizeof(out->pattern), "%s", pat.c_str());
-  out->ms = -1;
-  out->err = -1;
+  out->ms = pred[0];
+  out->err = pred[1];
   return CAPS_OK;
 }
 

@@ -363,10 +363,10 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
 // ------------------------------------------------------------------ INT8 GEMM, large tiles
 // 256 x 256 block tiles, 8 waves (2 per SIMD) of 128 x 64 (32 accumulator
 // tiles of 16 x 16 per wave), K-steps of 128.  Operands go global -> LDS
-// directly (global_load_lds_dwordx4: each lane's 16 bytes land at the wave's
-// LDS base + 16 * lane), double-buffered, so no registers hold the next
-// stage.  Per MAC this reads 3/4 of the LDS bytes and half the L2 bytes of
-// the 128 x 128 kernel above.  Needs m0 % 256 == 0.
+// directly (MUBUF buffer_load ... lds: each lane's 16 bytes land at the
+// wave's LDS base + 16 * lane), double-buffered, so no registers hold the
+// next stage.  Per MAC this reads 3/4 of the LDS bytes and half the L2 bytes
+// of the 128 x 128 kernel above.  Needs m0 % 256 == 0.
 //
 // LDS stage layout per operand: 256 rows x 8 chunks of 16 bytes (k), no
 // padding; chunk c of row r sits in slot c ^ swz(r), swz(r) = ((r >> 2) & 3)
@@ -375,18 +375,40 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
 // lie in one half of the wave, and addresses conflict modulo 512 bytes.  With
 // this swizzle the lanes of every such 16-lane set (rows l16 of a 16-row
 // group, chunks lk and lk' of one half-wave) hit distinct 16-byte slots.
+//
+// Latency and ordering, each choice measured on 49 x 14 GEMMs of 2048^3:
+//  * the LDS-bound loads are MUBUF, not global_load_lds: with the FLAT-class
+//    form in the loop the compiler waits for every LDS read with
+//    lgkmcnt(0); with MUBUF it waits for exactly the fragment an MFMA needs;
+//  * the barrier closing a K-step is a bare s_barrier after this wave's own
+//    s_waitcnt vmcnt: __syncthreads adds a release fence that waits for every
+//    outstanding load, including the L2 prefetch below;
+//  * the second half-step's fragments are read while the first half-step's
+//    MFMAs run (sched_group_barrier interleave);
+//  * an L2 prefetch two K-steps ahead: the 32 tiles one XCD runs at a time
+//    form an 8 x 4 patch of one problem (xcd_tile, tile_coords), so each of
+//    its blocks touches 1/32 of the patch's next A and B slabs (4 bytes per
+//    128-byte line) and the whole XCD finds them in L2;
+//  * the next tile's first stage is loaded before the epilogue.
 constexpr int JBM = 256, JBK = 128, JT = 512;
 constexpr int JSTAGE = JBM * JBK;  // 32 KB per operand per stage
+constexpr int JPF = 2;             // prefetch distance, K-steps
 
 __device__ __forceinline__ int jswz(int r) { return ((r >> 2) & 3) << 1; }
 
 typedef __attribute__((address_space(3))) void *lds_ptr;
 
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const int8_t *p, long long bytes) {
+  // raw buffer over one operand's quadtree (bytes < 2^31 for m <= 46340)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t *>(p), 0, (int)bytes, 0x00020000);
+}
+
 // one stage: wave w loads rows 8 (8 j + w) .. + 7 of A and of B, 8 lanes per
-// 128-byte row; for those rows swz(r) = (((w & 1) << 1) | (rl >> 2)) << 1
-__device__ __forceinline__ void stage_load(const int8_t *__restrict__ Ab,
-                                           const int8_t *__restrict__ Bb, long long ld,
-                                           int8_t *As, int8_t *Bs, int wave, int lane) {
+// 128-byte row; for those rows swz(r) = (((w & 1) << 1) | (rl >> 2)) << 1.
+// aoff / boff: byte offsets of the slab origin in the operand.
+__device__ __forceinline__ void stage_load(__amdgpu_buffer_rsrc_t ra, __amdgpu_buffer_rsrc_t rb,
+                                           int aoff, int boff, int ld, int8_t *As, int8_t *Bs,
+                                           int wave, int lane) {
   const int rl = lane >> 3;
   // chunk this lane fetches (same for every j): slot (lane & 7) holds chunk slot ^ swz(r)
   const int c = (lane & 7) ^ ((((wave & 1) << 1) | (rl >> 2)) << 1);
@@ -394,9 +416,48 @@ __device__ __forceinline__ void stage_load(const int8_t *__restrict__ Ab,
   for (int j = 0; j < 4; j++) {
     const int r = j * 64 + wave * 8 + rl;
     const int slot0 = (j * 8 + wave) * 64;  // first 16-byte slot of this wave's 1 KB
-    __builtin_amdgcn_global_load_lds(Ab + r * ld + c * 16, (lds_ptr)(As + slot0 * 16), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(Bb + r * ld + c * 16, (lds_ptr)(Bs + slot0 * 16), 16, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_ptr)(As + slot0 * 16), 16, r * ld + c * 16,
+                                             aoff, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_ptr)(Bs + slot0 * 16), 16, r * ld + c * 16,
+                                             boff, 0, 0);
   }
+}
+
+// block barrier for LDS stages filled by this wave's own loads: the caller
+// has waited for them (vmcnt) and consumed every LDS read of the stage it
+// leaves; the empty asm statements keep memory operations on their side
+__device__ __forceinline__ void stage_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+struct BigTile {
+  const int8_t *Ap, *Bp;
+  int8_t *Cp;
+  int mi, c14, row0, col0, ti, tj;
+  float mf, invf;
+};
+
+__device__ __forceinline__ BigTile big_tile(const RGemm *__restrict__ tab, const TileMap &map,
+                                            int tile, int nmod,
+                                            const OzakiConsts *__restrict__ kc) {
+  BigTile t;
+  int prob, ti, tj;
+  tile_coords(map, tile, prob, ti, tj);
+  const int leaf = prob / nmod, l = prob - leaf * nmod;
+  t.Ap = tab[leaf].a + l * tab[leaf].sa;  // row-major leaves
+  t.Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major leaves
+  t.Cp = tab[leaf].c + l * tab[leaf].sc;
+  t.mi = kc->m[l];
+  t.mf = (float)t.mi;
+  t.invf = 1.0f / t.mf;
+  t.c14 = kc->c14[l];
+  t.row0 = ti * JBM;
+  t.col0 = tj * JBM;
+  t.ti = ti;
+  t.tj = tj;
+  return t;
 }
 
 __global__ __launch_bounds__(JT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
@@ -406,94 +467,132 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
   __shared__ __attribute__((aligned(1024))) int8_t A1[JSTAGE];
   __shared__ __attribute__((aligned(1024))) int8_t B0[JSTAGE];
   __shared__ __attribute__((aligned(1024))) int8_t B1[JSTAGE];
+  __shared__ __attribute__((aligned(256))) unsigned PF[8][64];  // prefetch sink
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves: rows wm * 128, columns wn * 64
   const int l16 = lane & 15, lk = lane >> 4;
   const int G = gridDim.x;
-  const int nk = m / JBK;
-  const bool small_k = m <= 16384;  // |sums| <= m 2^14 < 2^28: the cheap exact reduction
+  const int nk = m / JBK;  // even: m0 % 256 == 0
+  // |sums| <= m 2^14 < 2^28 for m <= 16384: the one-pass exact reduction
+  // res_byte128 (sums start at 128); larger K takes the general one
+  const bool small_k = m <= 16384;
+  const int bias = small_k ? 128 : 0;
   // read offsets (bytes) of this lane's row l16, chunk lk (kh = 0) and lk + 4 (kh = 1)
   const int s16 = jswz(l16);
   const int rd0 = l16 * JBK + ((lk ^ s16) << 4);
   const int rd1 = l16 * JBK + (((lk + 4) ^ s16) << 4);
   const int aoff = wm * 128 * JBK, boff = wn * 64 * JBK;
+  const long long mm0 = (long long)m0 * m0;
+  const long long opb = (long long)m * m;  // bytes of one operand quadtree
+  // the prefetch needs the 8 x 4 patch order of tile_coords
+  const bool pf_wave = map.patch && wave < 2;
 
-  for (int round = 0; round * G < map.total; round++) {
-    const int tile = round * G + xcd_tile(blockIdx.x, G);
-    if (tile >= map.total) break;
-    int prob, ti, tj;
-    tile_coords(map, tile, prob, ti, tj);
-    const int leaf = prob / nmod, l = prob - leaf * nmod;
-    const int8_t *__restrict__ Ap = tab[leaf].a + l * tab[leaf].sa;  // row-major leaves
-    const int8_t *__restrict__ Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major leaves
-    int8_t *__restrict__ Cp = tab[leaf].c + l * tab[leaf].sc;
-    const int mi = kc->m[l];
-    const float mf = (float)mi, invf = 1.0f / mf;
-    const int c14 = kc->c14[l];
-    const int row0 = ti * JBM, col0 = tj * JBM;
+  int tile = xcd_tile(blockIdx.x, G);
+  if (tile >= map.total) return;
+  BigTile t = big_tile(tab, map, tile, nmod, kc);
+  // offsets of row `row` / column `col` at k in the quadtree operands
+  auto a_off = [&](int row, int k) -> long long {
+    return quad_leaf(row / m0, k / m0, d) * mm0 + (long long)(row % m0) * m0 + (k % m0);
+  };
+  auto b_off = [&](int col, int k) -> long long {
+    return quad_leaf(k / m0, col / m0, d) * mm0 + (long long)(col % m0) * m0 + (k % m0);
+  };
+  // waves 0 and 1: this block's 1/32 share of the patch's A / B slab at k
+  auto prefetch = [&](const BigTile &c, int k) {
+    const int w = (c.ti & 7) + 8 * (c.tj & 3);
+    const int pr0 = (c.ti & ~7) * JBM, pc0 = (c.tj & ~3) * JBM;
+    const int8_t *p = wave == 0 ? c.Ap + a_off(pr0 + w * 64 + lane, k)
+                                : c.Bp + b_off(pc0 + w * 32 + (lane & 31), k);
+    const int8_t *base = wave == 0 ? c.Ap : c.Bp;
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(plane_rsrc(base, opb), (lds_ptr)&PF[wave][0], 4,
+                                             (int)(p - base), 0, 0, 0);
+  };
+  auto load = [&](const BigTile &c, int k, int8_t *As, int8_t *Bs) {
+    stage_load(plane_rsrc(c.Ap, opb), plane_rsrc(c.Bp, opb), (int)a_off(c.row0, k),
+               (int)b_off(c.col0, k), m0, As, Bs, wave, lane);
+  };
+  load(t, 0, A0, B0);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
 
-    // small K: the sums start at 128 for the one-pass reduction res_byte128
-    const int bias = small_k ? 128 : 0;
+  for (int round = 0;; round++) {
     v4i acc[8][4];
 #pragma unroll
     for (int r = 0; r < 8; r++)
 #pragma unroll
       for (int c = 0; c < 4; c++) acc[r][c] = v4i{bias, bias, bias, bias};
-
-    const long long mm0 = (long long)m0 * m0;
-    const long long arow = (long long)(row0 % m0) * m0, bcol = (long long)(col0 % m0) * m0;
-    // offsets of the (row0, k) slice of A and the (k, col0) slice of B
-    // (d == 0: plain leaves, no quadtree arithmetic in the k loop)
-    auto a_off = [&](int k) -> long long {
-      return d == 0 ? arow + k : quad_leaf(row0 / m0, k / m0, d) * mm0 + arow + (k % m0);
-    };
-    auto b_off = [&](int k) -> long long {
-      return d == 0 ? bcol + k : quad_leaf(k / m0, col0 / m0, d) * mm0 + bcol + (k % m0);
-    };
-    // (the previous tile's k loop ended with a barrier after its last LDS reads)
-    stage_load(Ap + a_off(0), Bp + b_off(0), m0, A0, B0, wave, lane);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
+    // scalar loads of the tile context complete here: a pending one would
+    // turn every LDS wait of the K loop into lgkmcnt(0)
+    __builtin_amdgcn_s_waitcnt(0xc07f);
     for (int kt = 0; kt < nk; kt++) {
       const bool odd = kt & 1;
-      if (kt + 1 < nk) {
-        const int k1 = (kt + 1) * JBK;
-        stage_load(Ap + a_off(k1), Bp + b_off(k1), m0, odd ? A0 : A1, odd ? B0 : B1, wave, lane);
+      if (kt + 1 < nk) load(t, (kt + 1) * JBK, odd ? A0 : A1, odd ? B0 : B1);
+      bool pf = false;
+      if (pf_wave && kt + JPF < nk) {
+        prefetch(t, (kt + JPF) * JBK);
+        pf = true;
       }
       const int8_t *Ab = (odd ? A1 : A0) + aoff;
       const int8_t *Bb = (odd ? B1 : B0) + boff;
+      v4i a0[8], b0[4], a1[8], b1[4];
 #pragma unroll
-      for (int kh = 0; kh < 2; kh++) {
-        const int rd = kh ? rd1 : rd0;
-        v4i a[8], b[4];
+      for (int c = 0; c < 4; c++) b0[c] = *reinterpret_cast<const v4i *>(Bb + rd0 + c * 16 * JBK);
 #pragma unroll
-        for (int r = 0; r < 8; r++) a[r] = *reinterpret_cast<const v4i *>(Ab + rd + r * 16 * JBK);
+      for (int r = 0; r < 8; r++) a0[r] = *reinterpret_cast<const v4i *>(Ab + rd0 + r * 16 * JBK);
 #pragma unroll
-        for (int c = 0; c < 4; c++) b[c] = *reinterpret_cast<const v4i *>(Bb + rd + c * 16 * JBK);
+      for (int c = 0; c < 4; c++) b1[c] = *reinterpret_cast<const v4i *>(Bb + rd1 + c * 16 * JBK);
 #pragma unroll
-        for (int r = 0; r < 8; r++)
+      for (int r = 0; r < 8; r++) a1[r] = *reinterpret_cast<const v4i *>(Ab + rd1 + r * 16 * JBK);
 #pragma unroll
-          for (int c = 0; c < 4; c++)
-            acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], b[c], acc[r][c], 0, 0, 0);
+      for (int r = 0; r < 8; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[r], b0[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[r], b1[c], acc[r][c], 0, 0, 0);
+      // order: the stage and prefetch loads, the 12 first-half reads, then the
+      // 12 second-half reads spread over the first 24 MFMAs, then the rest
+      __builtin_amdgcn_sched_group_barrier(0x020, 9, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);
+#pragma unroll
+      for (int i = 0; i < 12; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
       }
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      __syncthreads();
+      __builtin_amdgcn_sched_group_barrier(0x008, 40, 0);
+      // this wave's stage loads are done; its prefetch (issued after them)
+      // may still be in flight
+      if (pf)
+        asm volatile("s_waitcnt vmcnt(1)" ::: "memory");
+      else
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      stage_barrier();
     }
-
+    // nk is even: the last K-step read A1 / B1 and A0 / B0 are free, so the
+    // next tile's first stage goes out before this tile's epilogue
+    const int next = (round + 1) * G + xcd_tile(blockIdx.x, G);
+    const bool has_next = next < map.total;
+    BigTile tn = t;
+    if (has_next) tn = big_tile(tab, map, next, nmod, kc);
+    if (has_next) load(tn, 0, A0, B0);
     // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v; the
     // lane's word of block (r, c) sits at Cb + 16 c m0 + 16 r
-    int8_t *Cb = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0 +
-                 (long long)((col0 % m0) + wn * 64 + l16) * m0 + (row0 % m0) + wm * 128 + 4 * lk;
+    int8_t *Cb = t.Cp + quad_leaf(t.row0 / m0, t.col0 / m0, d) * mm0 +
+                 (long long)((t.col0 % m0) + wn * 64 + l16) * m0 + (t.row0 % m0) + wm * 128 +
+                 4 * lk;
     if (small_k) {
-      const int c14m = c14 - 16384;
-      const float cl = 0.5f - 128.0f * invf;
+      const int c14m = t.c14 - 16384;
+      const float cl = 0.5f - 128.0f * t.invf;
 #pragma unroll
       for (int r = 0; r < 8; r++)
 #pragma unroll
         for (int c = 0; c < 4; c++) {
           unsigned v = 0;
 #pragma unroll
-          for (int q = 0; q < 4; q++) v = res_byte128(acc[r][c][q], c14m, mf, invf, cl, q, v);
+          for (int q = 0; q < 4; q++) v = res_byte128(acc[r][c][q], c14m, t.mf, t.invf, cl, q, v);
           *(gu32 *)(Cb + (long long)(16 * c) * m0 + 16 * r) = v ^ 0x80808080u;
         }
     } else {
@@ -503,10 +602,15 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
         for (int c = 0; c < 4; c++) {
           unsigned v = 0;
 #pragma unroll
-          for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
-          *reinterpret_cast<unsigned *>(Cb + (long long)(16 * c) * m0 + 16 * r) = v;
+          for (int q = 0; q < 4; q++)
+            v |= (unsigned)(mod_i32(acc[r][c][q], t.mi, t.invf) & 0xff) << (8 * q);
+          *(gu32 *)(Cb + (long long)(16 * c) * m0 + 16 * r) = v;
         }
     }
+    if (!has_next) break;
+    t = tn;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    stage_barrier();
   }
 }
 
@@ -766,8 +870,9 @@ hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, i
   const int m0 = m >> levels;
   if (levels < 0 || (m0 << levels) != m) return hipErrorInvalidValue;
   bool ok;
-  if (m0 % JBM == 0) {
-    // large tiles: one block (8 waves, 128 KB of LDS) per CU
+  if (m0 % JBM == 0 && (long long)m * m < (1LL << 31)) {
+    // large tiles: one block (8 waves, 128 KB of LDS) per CU; buffer offsets
+    // are 32-bit, so an operand quadtree stays below 2 GiB
     TileMap map = make_tilemap(m, JBM, (long long)count * nmod, ok);
     if (!ok) return hipErrorInvalidValue;
     const int G = persistent_grid(1, device_cu_count(), map.total);

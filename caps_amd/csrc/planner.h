@@ -20,8 +20,9 @@ std::string choose_pattern(const caps_desc &d, int mode, int moduli, long long b
 int choose_mode(const caps_desc &d);
 
 // The learned planner (tune_plan.cpp): false when no model is installed or no
-// candidate is predicted to meet max_err.
+// candidate is predicted to meet max_err.  pred (may be NULL) receives the
+// model's predicted milliseconds and normwise error of the choice.
 bool mlp_planner_choose(const caps_desc &d, int mask, double max_err, double spread, int &mode,
-                        int &moduli, int &cutoff, std::string &pat);
+                        int &moduli, int &cutoff, std::string &pat, double *pred = nullptr);
 
 }  // namespace caps

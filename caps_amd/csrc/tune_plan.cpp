@@ -101,7 +101,7 @@ ModelSlot &model() {
 }  // namespace
 
 bool mlp_planner_choose(const caps_desc &d, int mask, double max_err, double spread, int &mode,
-                        int &moduli, int &cutoff, std::string &pat) {
+                        int &moduli, int &cutoff, std::string &pat, double *pred) {
   ModelSlot &ms = model();
   std::lock_guard<std::mutex> g(ms.mu);
   if (!ms.m) return false;
@@ -119,16 +119,21 @@ bool mlp_planner_choose(const caps_desc &d, int mask, double max_err, double spr
   const float *tm = ms.norm + 2 * MLP_IN, *ts = tm + MLP_OUT;
   const double target = std::log10(max_err);
   int best = -1;
-  double best_t = INFINITY;
+  double best_t = INFINITY, best_e = 0;
   for (int i = 0; i < B; i++) {
     const double lt = Y[(size_t)i * MLP_OUT] * ts[0] + tm[0];      // log2(ms)
     const double le = Y[(size_t)i * MLP_OUT + 1] * ts[1] + tm[1];  // log10(error)
     if (le <= target - 0.25 && lt < best_t) {  // a quarter-decade of safety on the error
       best_t = lt;
+      best_e = le;
       best = i;
     }
   }
   if (best < 0) return false;
+  if (pred) {
+    pred[0] = std::exp2(best_t);
+    pred[1] = std::pow(10.0, best_e);
+  }
   mode = cands[best].mode;
   moduli = cands[best].moduli ? cands[best].moduli : OZ_DEFAULT_MODULI;
   cutoff = cands[best].cutoff;

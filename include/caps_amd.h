@@ -241,9 +241,10 @@ int caps_autotune(const double *dA, const double *dB, double *dC, const caps_des
 /* The leaf mode, moduli, cutoff and pattern that a call with pattern NULL
  * and these options would run (opts NULL: FP64 with the memory-budget
  * pattern rule above; CAPS_MODE_AUTO: the tuning table, then the learned
- * model, then the default), without multiplying.  Needs no device unless the
- * learned model is consulted.  multiplyInternal's automatic choice,
- * multiply.cpp:43-66. */
+ * model, then the default), without multiplying; out->ms and out->err are the
+ * learned model's predicted time and error when it made the choice, else -1.
+ * Needs no device unless the learned model is consulted.  multiplyInternal's
+ * automatic choice, multiply.cpp:43-66. */
 int caps_plan_choice(const caps_desc *desc, const caps_opts *opts, caps_candidate *out);
 /* The error target and allowed modes that CAPS_MODE_AUTO calls look up. */
 int caps_planner_config(double max_err, int mode_mask, double exponent_spread);

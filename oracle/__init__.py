@@ -34,6 +34,47 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+REF_TREE = "/root/reference"
+REF_BENCH = os.path.join(_HERE, "_ref", "randombenchmark")
+
+
+def build_reference(ref: str = REF_TREE) -> str | None:
+    """Compile the reference's own randombenchmark from its sources with the
+    committed recipe (ref.mk: MPICH wrapper, MKL dgemm_) into oracle/_ref/.
+    Only where the reference tree exists; elsewhere the prebuilt binary (if
+    any) is used as is.  Returns the binary's path or None."""
+    if os.path.isdir(ref):
+        r = subprocess.run(["make", "-s", "-B", "-f", os.path.join(_HERE, "ref.mk"), f"REF={ref}"],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            print("oracle: reference build failed:\n" + r.stderr[-2000:])
+            return None
+    return REF_BENCH if os.path.exists(REF_BENCH) else None
+
+
+def run_reference_benchmark(n: int, nrec: int, pattern: str, threads: int,
+                            timeout: float = 300.0) -> dict | None:
+    """Run oracle/_ref/randombenchmark (one MPI process, MKL with `threads`
+    GNU OpenMP threads) and parse its printCounters report
+    (counters.cpp:61-74).  None if the binary is absent or fails."""
+    if not os.path.exists(REF_BENCH):
+        return None
+    env = dict(os.environ, MKL_THREADING_LAYER="GNU", MKL_NUM_THREADS=str(threads),
+               OMP_NUM_THREADS=str(threads))
+    try:
+        r = subprocess.run([REF_BENCH, "-s", str(n), "-r", str(nrec), "-p", pattern,
+                            "-m", str(max(1536, 3 * n * n * 8 // (1 << 20) * 4))],
+                           capture_output=True, text=True, env=env, timeout=timeout)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    m = re.search(r"time ([0-9.eE+-]+) actual Gflop/s ([0-9.eE+-]+) per process effective "
+                  r"Gflop/s ([0-9.eE+-]+)", r.stdout)
+    if r.returncode != 0 or not m:
+        return None
+    return {"time_s": float(m.group(1)), "actual_gflops": float(m.group(2)),
+            "effective_gflops": float(m.group(3)), "report": r.stdout.strip().splitlines()[-4:]}
+
+
 def lib():
     global _lib
     if _lib is None:
