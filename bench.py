@@ -231,7 +231,8 @@ def uneven_batch_section(args, torch, caps_amd, measure=False, compare_single=Fa
                             (caps_amd.MODE_CRT_INT8, "split", 1e-14)):
         t0 = time.perf_counter()
         plans = batch_plan.choose_plans(sizes, mode, max_err=err, spread=0.0,
-                                        measure_with=torch if measure else None)
+                                        measure_with=torch if measure else None,
+                                        moduli=None if mode == caps_amd.MODE_FP64 else args.moduli)
         caps_amd.release()
         log(f"[bench] batch {name}: plans for {len(plans)} sizes in "
             f"{time.perf_counter() - t0:.1f} s")

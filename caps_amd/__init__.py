@@ -39,6 +39,7 @@ FLAG_NO_GRAPH = 1
 FLAG_REFERENCE_AUTO = 2
 FLAG_TIMERS = 4
 FLAG_NO_FUSION = 8
+FLAG_SMALL_TILE = 16
 DEFAULT_MODULI = 14
 # phase timers (counters.h:13-18 order, then the split mode's phases)
 TIMER_NAMES = ("comm", "add", "mul", "rearrange", "comm_summa", "split", "recon", "unused")
@@ -84,12 +85,13 @@ class Counters(ctypes.Structure):
 class Candidate(ctypes.Structure):
     """caps_candidate: one configuration of the autotuner / planner."""
     _fields_ = [("mode", ctypes.c_int), ("moduli", ctypes.c_int), ("cutoff", ctypes.c_int),
-                ("reserved", ctypes.c_int), ("pattern", ctypes.c_char * 32),
+                ("tile", ctypes.c_int), ("pattern", ctypes.c_char * 32),
                 ("ms", ctypes.c_double), ("err", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {"mode": self.mode, "moduli": self.moduli, "cutoff": self.cutoff,
-                "pattern": self.pattern.decode(), "ms": self.ms, "err": self.err}
+                "tile": self.tile, "pattern": self.pattern.decode(), "ms": self.ms,
+                "err": self.err}
 
 
 MASK_FP64 = 1 << MODE_FP64

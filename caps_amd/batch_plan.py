@@ -57,12 +57,13 @@ _MASKS = {caps_amd.MODE_FP64: caps_amd.MASK_FP64,
 
 
 def choose_plans(sizes, mode: int, max_err: float = 1e-14, spread: float = 0.0,
-                 measure_with=None, reps: int = 3) -> dict:
+                 measure_with=None, reps: int = 3, moduli: int | None = None) -> dict:
     """One Plan per distinct size.  The model (if installed, e.g. by
     planner_train.load_model) scores every candidate of every depth; the
     predicted-fastest depth wins.  With measure_with=torch the model's choice
     at each depth is also timed on the device and the measured-fastest wins.
-    Split modes share one modulus count in a batch: the largest chosen one."""
+    Split modes share one modulus count in a batch: `moduli` if given, else
+    the largest chosen one."""
     caps_amd.planner_config(max_err, _MASKS[mode], spread)
     out = {}
     for n in sorted(set(int(x) for x in sizes)):
@@ -75,7 +76,7 @@ def choose_plans(sizes, mode: int, max_err: float = 1e-14, spread: float = 0.0,
                 continue
             if c["mode"] != mode or c["predicted_ms"] < 0:
                 continue  # no model, or it found nothing for this mode
-            p = Plan(r, c["pattern"], c["cutoff"], c["moduli"], c["predicted_ms"])
+            p = Plan(r, c["pattern"], c["cutoff"], moduli or c["moduli"], c["predicted_ms"])
             if measure_with is not None:
                 p.measured_ms = _time_plan(measure_with, n, p, mode, reps)
             key = p.measured_ms if measure_with is not None else p.predicted_ms
@@ -87,7 +88,7 @@ def choose_plans(sizes, mode: int, max_err: float = 1e-14, spread: float = 0.0,
         for p in out.values():
             p.moduli = 0
     else:
-        k = max(p.moduli for p in out.values())
+        k = moduli if moduli else max(p.moduli for p in out.values())
         for p in out.values():
             p.moduli = k
     return out

@@ -230,8 +230,8 @@ int make_plan_locked(State &s, const PlanKey &key, F build, Plan *&out) {
 
 int get_plan_locked(State &s, const double *A, const double *B, double *C, const caps_desc &d,
                     const std::string &pat, int mode, int moduli, int cutoff, bool fuse,
-                    hipStream_t stream, Plan *&out) {
-  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff, fuse, std::string(), stream};
+                    hipStream_t stream, int tile, Plan *&out) {
+  PlanKey key{A, B, C, d, pat, mode, moduli, cutoff, fuse, std::string(), stream, tile};
   for (auto &p : s.plans)
     if (p->key == key) {
       out = p.get();
@@ -241,7 +241,7 @@ int get_plan_locked(State &s, const double *A, const double *B, double *C, const
   return make_plan_locked(
       s, key,
       [&](Plan &pl, void *work) {
-        return build_plan(pl, A, B, C, d, pat, mode, moduli, cutoff, work, fuse);
+        return build_plan(pl, A, B, C, d, pat, mode, moduli, cutoff, work, fuse, tile);
       },
       out);
 }
@@ -269,14 +269,16 @@ int capture_graph(Plan &p) {
 // CAPS_MODE_AUTO: the tuning table, then the learned model, then a default
 // (residue-domain split when allowed: it meets a 1e-14 bar; else FP64)
 int auto_choose(State &s, const caps_desc &d, const char *pattern, int &mode, int &moduli,
-                int &cutoff, std::string &pat, double *pred = nullptr) {
+                int &cutoff, std::string &pat, double *pred = nullptr, int *tile = nullptr) {
   TuneEntry e;
+  if (tile) *tile = 0;
   if (tune_lookup(d, s.auto_mask, s.auto_max_err, e)) {
     // a table entry goes through the same checks as a caller's options
     caps_opts te{e.c.mode, e.c.moduli, 0, e.c.cutoff};
     int rc = resolve_mode(d, &te, mode, moduli);
     if (rc) return rc;
     cutoff = e.c.cutoff;
+    if (tile) *tile = e.c.tile ? 1 : 0;
     if (!pattern && (int)strlen(e.c.pattern) >= d.nrec)
       return resolve_pattern(d, e.c.pattern, &te, pat, mode, moduli, s.available_doubles,
                              s.budget_set);
@@ -300,10 +302,13 @@ int multiply_device_locked(State &s, const double *dA, const double *dB, double 
   if (rc) return rc;
   if (!dA || !dB || !dC) return fail(CAPS_EINVAL, "NULL matrix pointer");
   int mode, moduli, cutoff;
+  int tile = (opts && (opts->flags & CAPS_FLAG_SMALL_TILE)) ? 1 : 0;
   std::string pat;
   if (opts && opts->mode == CAPS_MODE_AUTO) {
-    rc = auto_choose(s, *desc, pattern, mode, moduli, cutoff, pat);
+    int tuned_tile = 0;
+    rc = auto_choose(s, *desc, pattern, mode, moduli, cutoff, pat, nullptr, &tuned_tile);
     if (rc) return rc;
+    tile |= tuned_tile;
     if (pattern) {  // the caller's pattern with the chosen leaf mode
       caps_opts o{mode, moduli, 0, cutoff};
       rc = resolve_pattern(*desc, pattern, &o, pat, mode, moduli, s.available_doubles,
@@ -324,7 +329,8 @@ int multiply_device_locked(State &s, const double *dA, const double *dB, double 
     return fail(CAPS_EUNSUPPORTED, "residue-domain mode supports nrec <= 7 (got %d)", desc->nrec);
   Plan *plan = nullptr;
   const bool fuse = !(opts && (opts->flags & CAPS_FLAG_NO_FUSION));
-  rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, cutoff, fuse, stream, plan);
+  rc = get_plan_locked(s, dA, dB, dC, *desc, pat, mode, moduli, cutoff, fuse, stream, tile,
+                       plan);
   if (rc) return rc;
   const bool timed = opts && (opts->flags & CAPS_FLAG_TIMERS);
   const bool use_graph = !timed && !(opts && (opts->flags & CAPS_FLAG_NO_GRAPH)) &&
@@ -590,8 +596,13 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
              items[i].pattern.c_str());
     key += buf;
   }
-  PlanKey pk{nullptr, nullptr, nullptr, descs[0], std::string(), mode, moduli, 0, true, key,
-             stream};
+  const int tile = (opts && (opts[0].flags & CAPS_FLAG_SMALL_TILE)) ? 1 : 0;
+  // merged groups (segmented launches) unless CAPS_FLAG_NO_FUSION, within the
+  // memory budget
+  const bool merge = !(opts && (opts[0].flags & CAPS_FLAG_NO_FUSION));
+  const size_t merge_budget = merge ? planner_budget_bytes(s.available_doubles, s.budget_set) : 0;
+  PlanKey pk{nullptr, nullptr, nullptr, descs[0], std::string(), mode, moduli, 0, merge, key,
+             stream, tile};
   Plan *plan = nullptr;
   for (auto &p : s.plans)
     if (p->key == pk) {
@@ -602,7 +613,7 @@ int caps_multiply_batch_device(int count, const double *const *dA, const double 
     rc = make_plan_locked(
         s, pk,
         [&](Plan &pl, void *work) {
-          return build_batch_plan(pl, items, mode, moduli, work);
+          return build_batch_plan(pl, items, mode, moduli, work, tile, merge_budget);
         },
         plan);
     if (rc) return rc;
@@ -669,7 +680,7 @@ int caps_measure(const double *dA, const double *dB, double *dC, const caps_desc
   rc = check_desc(desc);
   if (rc) return rc;
   hipStream_t hs = (hipStream_t)stream;
-  caps_opts o{cand->mode, cand->moduli, 0, cand->cutoff};
+  caps_opts o{cand->mode, cand->moduli, cand->tile ? CAPS_FLAG_SMALL_TILE : 0, cand->cutoff};
   std::string pat;
   if (cand->pattern[0]) {
     pat = cand->pattern;
@@ -719,8 +730,11 @@ int caps_plan_choice(const caps_desc *desc, const caps_opts *opts, caps_candidat
   int mode, moduli, cutoff;
   std::string pat;
   double pred[2] = {-1.0, -1.0};
+  int tile = (opts && (opts->flags & CAPS_FLAG_SMALL_TILE)) ? 1 : 0;
   if (opts && opts->mode == CAPS_MODE_AUTO) {
-    rc = auto_choose(s, *desc, nullptr, mode, moduli, cutoff, pat, pred);
+    int tuned_tile = 0;
+    rc = auto_choose(s, *desc, nullptr, mode, moduli, cutoff, pat, pred, &tuned_tile);
+    tile |= tuned_tile;
   } else {
     rc = resolve_mode(*desc, opts, mode, moduli);
     cutoff = opts ? opts->cutoff : 0;
@@ -733,6 +747,7 @@ int caps_plan_choice(const caps_desc *desc, const caps_opts *opts, caps_candidat
   out->mode = mode;
   out->moduli = moduli;
   out->cutoff = cutoff;
+  out->tile = tile;
   snprintf(out->pattern, sizeof(out->pattern), "%s", pat.c_str());
   out->ms = pred[0];
   out->err = pred[1];

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kernels.h"
 #include "ozaki.h"
 
 namespace caps {
@@ -53,6 +54,27 @@ hipError_t launch_crt_step(int step, const CrtMats *d_mats, const CrtMats &host_
 // Winograd add step `op` (EwOp) on residue planes, all nmod moduli
 hipError_t launch_residue_ew(int op, const RPtrs *d_tab, int count, long long n, int nmod,
                              bool vec4, const OzakiConsts *d_consts, hipStream_t s);
+
+// ---- segmented launches over the items of an uneven batch (kernels.h VSeg)
+// the kernels of the scale, split and reconstruct steps
+enum CrtKernel : int {
+  CRT_ROWPART = 0,
+  CRT_COLPART = 1,
+  CRT_FINALIZE = 2,
+  CRT_SPLIT_A = 3,
+  CRT_SPLIT_B = 4,
+  CRT_RECON = 5
+};
+// the grid kernel k has for one item alone (gx x gy) and its template flag
+// (column-major leaves / aligned sizes); items merged in one launch share it
+bool crt_grid(int k, const CrtMats &h, int &gx, int &gy);
+hipError_t launch_crt_multi(int k, bool flag, int np, const char *d_tables, const VSeg *d_segs,
+                            int nseg, int blocks, const OzakiConsts *d_consts, hipStream_t s);
+// residue Winograd step over segments (gx blocks x entries * moduli, aux = moduli)
+void rew_grid(int count, long long n, int nmod, bool vec4, int &gx, int &gy);
+hipError_t launch_residue_ew_multi(int op, const char *d_tables, const VSeg *d_segs, int nseg,
+                                   int blocks, bool vec4, const OzakiConsts *d_consts,
+                                   hipStream_t s);
 
 // chunks of a leaf row used by the row-partials reduction
 inline int crt_row_chunks(int m) {

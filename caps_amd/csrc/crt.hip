@@ -15,11 +15,11 @@ constexpr int T = 256;
 // partial (e, s) of row i of A over columns [lj0, lj1) of leaf column block bj.
 // PLAIN: column-major leaves (bs == 1 or bs == m), 8 columns per step.
 template <bool PLAIN>
-__global__ __launch_bounds__(T) void crt_rowpart_kernel(const CrtMats *__restrict__ M) {
+__device__ __forceinline__ void rowpart_body(const CrtMats *__restrict__ M, int bx, int by) {
   const int n = M->n, m = M->m, nrec = M->nrec, bs = M->bs, ch = M->chunks;
-  const int i = blockIdx.x * T + threadIdx.x;
+  const int i = bx * T + threadIdx.x;
   if (i >= n) return;
-  const int c = blockIdx.y;  // bj * chunks + k
+  const int c = by;  // bj * chunks + k
   const int bj = c / ch, k = c - bj * ch;
   const int per = (m + ch - 1) / ch;
   const int lj0 = k * per, lj1 = min(m, lj0 + per);
@@ -45,12 +45,17 @@ __global__ __launch_bounds__(T) void crt_rowpart_kernel(const CrtMats *__restric
   reinterpret_cast<double2 *>(M->pa)[(long long)c * n + i] = make_double2(s, (double)e);
 }
 
+template <bool PLAIN>
+__global__ __launch_bounds__(T) void crt_rowpart_kernel(const CrtMats *__restrict__ M) {
+  rowpart_body<PLAIN>(M, blockIdx.x, blockIdx.y);
+}
+
 // partial of column j of B over leaf row block bi: one wave, fixed butterfly.
 // PLAIN: column-major leaves, each lane takes 8 rows (stride 64) per step.
 template <bool PLAIN>
-__global__ __launch_bounds__(T) void crt_colpart_kernel(const CrtMats *__restrict__ M) {
+__device__ __forceinline__ void colpart_body(const CrtMats *__restrict__ M, int bx) {
   const int n = M->n, m = M->m, nrec = M->nrec, bs = M->bs;
-  const long long w = (long long)blockIdx.x * (T / 64) + (threadIdx.x >> 6);
+  const long long w = (long long)bx * (T / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int nblk = 1 << nrec;
   if (w >= (long long)n * nblk) return;
@@ -83,10 +88,15 @@ __global__ __launch_bounds__(T) void crt_colpart_kernel(const CrtMats *__restric
   if (lane == 0) reinterpret_cast<double2 *>(M->pb)[(long long)bi * n + j] = make_double2(s, (double)e);
 }
 
-__global__ __launch_bounds__(T) void crt_finalize_kernel(const CrtMats *__restrict__ M,
-                                                         const OzakiConsts *__restrict__ kc) {
+template <bool PLAIN>
+__global__ __launch_bounds__(T) void crt_colpart_kernel(const CrtMats *__restrict__ M) {
+  colpart_body<PLAIN>(M, blockIdx.x);
+}
+
+__device__ __forceinline__ void finalize_body(const CrtMats *__restrict__ M,
+                                              const OzakiConsts *__restrict__ kc, int bx) {
   const int n = M->n;
-  const int i = blockIdx.x * T + threadIdx.x;
+  const int i = bx * T + threadIdx.x;
   if (i >= n) return;
   const int nblk = 1 << M->nrec;
   const double2 *pa = reinterpret_cast<const double2 *>(M->pa);
@@ -107,20 +117,25 @@ __global__ __launch_bounds__(T) void crt_finalize_kernel(const CrtMats *__restri
   M->sb[i] = scale_exponent(e, s, kc->L);
 }
 
+__global__ __launch_bounds__(T) void crt_finalize_kernel(const CrtMats *__restrict__ M,
+                                                         const OzakiConsts *__restrict__ kc) {
+  finalize_body(M, kc, blockIdx.x);
+}
+
 // ------------------------------------------------------------------ split
 // A leaf L -> row-major residue leaves; 64 x 64 tiles transposed through LDS.
 // FAST: m % 64 == 0 and the leaf is column-major (bs == 1 or bs == m).
 template <bool FAST>
-__global__ __launch_bounds__(T) void crt_split_a_kernel(const CrtMats *__restrict__ M,
-                                                        const OzakiConsts *__restrict__ kc) {
-  __shared__ double Ts[64][65];  // [col][row]
+__device__ __forceinline__ void split_a_body(const CrtMats *__restrict__ M,
+                                             const OzakiConsts *__restrict__ kc, int bx, int by,
+                                             double (&Ts)[64][65]) {
   const int m = M->m, nmod = M->nmod, bs = M->bs;
-  const int L = blockIdx.y;
+  const int L = by;
   int bi, bj;
   leaf_coords(L, M->nrec, bi, bj);
   const long long lbase = (long long)L * m * m;
   const int tiles = (m + 63) / 64;
-  const int r0 = (blockIdx.x % tiles) * 64, k0 = (blockIdx.x / tiles) * 64;
+  const int r0 = (bx % tiles) * 64, k0 = (bx / tiles) * 64;
   for (int idx = threadIdx.x; idx < 64 * 64; idx += T) {
     const int r = idx & 63, c = idx >> 6;
     const int li = r0 + r, lj = k0 + c;
@@ -164,21 +179,28 @@ __global__ __launch_bounds__(T) void crt_split_a_kernel(const CrtMats *__restric
   }
 }
 
+template <bool FAST>
+__global__ __launch_bounds__(T) void crt_split_a_kernel(const CrtMats *__restrict__ M,
+                                                        const OzakiConsts *__restrict__ kc) {
+  __shared__ double Ts[64][65];  // [col][row]
+  split_a_body<FAST>(M, kc, blockIdx.x, blockIdx.y, Ts);
+}
+
 // B leaf L -> column-major residue leaves; thread per 16 consecutive rows.
 // FAST: m % 16 == 0 and the leaf is column-major.
 template <bool FAST>
-__global__ __launch_bounds__(T) void crt_split_b_kernel(const CrtMats *__restrict__ M,
-                                                        const OzakiConsts *__restrict__ kc) {
+__device__ __forceinline__ void split_b_body(const CrtMats *__restrict__ M,
+                                             const OzakiConsts *__restrict__ kc, int bx, int by,
+                                             int gx) {
   const int m = M->m, nmod = M->nmod, bs = M->bs;
-  const int L = blockIdx.y;
+  const int L = by;
   int bi, bj;
   leaf_coords(L, M->nrec, bi, bj);
   const long long lbase = (long long)L * m * m;
   const long long mm = (long long)m * m;
   const long long plane = (long long)M->n * M->n;
   const long long units = FAST ? mm / 16 : mm;
-  for (long long u = (long long)blockIdx.x * T + threadIdx.x; u < units;
-       u += (long long)gridDim.x * T) {
+  for (long long u = (long long)bx * T + threadIdx.x; u < units; u += (long long)gx * T) {
     if (FAST) {
       const long long e0 = u * 16;
       const int lj = (int)(e0 / m);
@@ -216,13 +238,20 @@ __global__ __launch_bounds__(T) void crt_split_b_kernel(const CrtMats *__restric
   }
 }
 
+template <bool FAST>
+__global__ __launch_bounds__(T) void crt_split_b_kernel(const CrtMats *__restrict__ M,
+                                                        const OzakiConsts *__restrict__ kc) {
+  split_b_body<FAST>(M, kc, blockIdx.x, blockIdx.y, gridDim.x);
+}
+
 // ------------------------------------------------------------------ reconstruct
 // C' residues (column-major leaves) -> C (CAPS layout, bs blocks)
 template <int NP, bool FAST>
-__global__ __launch_bounds__(T) void crt_recon_kernel(const CrtMats *__restrict__ M,
-                                                      const OzakiConsts *__restrict__ kc) {
+__device__ __forceinline__ void recon_body(const CrtMats *__restrict__ M,
+                                           const OzakiConsts *__restrict__ kc, int bx, int by,
+                                           int gx) {
   const int m = M->m, nmod = M->nmod, bs = M->bs;
-  const int L = blockIdx.y;
+  const int L = by;
   int bi, bj;
   leaf_coords(L, M->nrec, bi, bj);
   const long long lbase = (long long)L * m * m;
@@ -230,8 +259,7 @@ __global__ __launch_bounds__(T) void crt_recon_kernel(const CrtMats *__restrict_
   const long long plane = (long long)M->n * M->n;
   const int per = FAST ? 4 : 1;
   const long long units = mm / per;
-  for (long long u = (long long)blockIdx.x * T + threadIdx.x; u < units;
-       u += (long long)gridDim.x * T) {
+  for (long long u = (long long)bx * T + threadIdx.x; u < units; u += (long long)gx * T) {
     const long long e0 = u * per;
     const int li = (int)(e0 % m), lj = (int)(e0 / m);
     const int sb = M->sb[bj * m + lj];
@@ -264,13 +292,37 @@ __global__ __launch_bounds__(T) void crt_recon_kernel(const CrtMats *__restrict_
   }
 }
 
+template <int NP, bool FAST>
+__global__ __launch_bounds__(T) void crt_recon_kernel(const CrtMats *__restrict__ M,
+                                                      const OzakiConsts *__restrict__ kc) {
+  recon_body<NP, FAST>(M, kc, blockIdx.x, blockIdx.y, gridDim.x);
+}
+
+// ------------------------------------------------------------------ segmented CRT steps
+// One launch per kernel over the items of an uneven batch: segment args are
+// CrtMats descriptions (crt_grid gives each item the grid it has alone).
+template <int K, bool F, int NP>
+__global__ __launch_bounds__(T) void crt_multi_kernel(const char *__restrict__ base,
+                                                      const VSeg *__restrict__ segs, int nseg,
+                                                      const OzakiConsts *__restrict__ kc) {
+  __shared__ double Ts[K == CRT_SPLIT_A ? 64 : 1][K == CRT_SPLIT_A ? 65 : 1];
+  const VSeg &S = segs[vseg_find(segs, nseg, blockIdx.x)];
+  const int l = blockIdx.x - S.blk0;
+  const int bx = l % S.gx, by = l / S.gx;
+  const CrtMats *M = reinterpret_cast<const CrtMats *>(base + S.arg);
+  if constexpr (K == CRT_ROWPART) rowpart_body<F>(M, bx, by);
+  if constexpr (K == CRT_COLPART) colpart_body<F>(M, bx);
+  if constexpr (K == CRT_FINALIZE) finalize_body(M, kc, bx);
+  if constexpr (K == CRT_SPLIT_A) split_a_body<F>(M, kc, bx, by, Ts);
+  if constexpr (K == CRT_SPLIT_B) split_b_body<F>(M, kc, bx, by, S.gx);
+  if constexpr (K == CRT_RECON) recon_body<NP, F>(M, kc, bx, by, S.gx);
+}
+
 // ------------------------------------------------------------------ residue Winograd steps
 template <int OP, bool VEC4>
-__global__ __launch_bounds__(T) void rew_kernel(const RPtrs *__restrict__ tab, long long n,
-                                                const OzakiConsts *__restrict__ kc) {
+__device__ __forceinline__ void rew_body(const RPtrs &P, long long n, int l, int bx, int gx,
+                                         const OzakiConsts *__restrict__ kc) {
   using Op = Ew<OP>;
-  const RPtrs &P = tab[blockIdx.y];
-  const int l = blockIdx.z;
   const int mi = kc->m[l], c22 = kc->c22[l];  // sums of at most 4 residues: |y| <= 512
   const int8_t *in[Op::NIN];
   int8_t *out[Op::NOUT];
@@ -278,10 +330,10 @@ __global__ __launch_bounds__(T) void rew_kernel(const RPtrs *__restrict__ tab, l
   for (int k = 0; k < Op::NIN; k++) in[k] = P.in[k] + l * P.sin[k];
 #pragma unroll
   for (int k = 0; k < Op::NOUT; k++) out[k] = P.out[k] + l * P.sout[k];
-  const long long stride = (long long)gridDim.x * T;
+  const long long stride = (long long)gx * T;
   if (VEC4) {
     const long long n4 = n >> 2;
-    for (long long i = (long long)blockIdx.x * T + threadIdx.x; i < n4; i += stride) {
+    for (long long i = (long long)bx * T + threadIdx.x; i < n4; i += stride) {
       unsigned w[Op::NIN];
 #pragma unroll
       for (int k = 0; k < Op::NIN; k++) w[k] = reinterpret_cast<const unsigned *>(in[k])[i];
@@ -302,7 +354,7 @@ __global__ __launch_bounds__(T) void rew_kernel(const RPtrs *__restrict__ tab, l
       for (int k = 0; k < Op::NOUT; k++) reinterpret_cast<unsigned *>(out[k])[i] = o[k];
     }
   } else {
-    for (long long i = (long long)blockIdx.x * T + threadIdx.x; i < n; i += stride) {
+    for (long long i = (long long)bx * T + threadIdx.x; i < n; i += stride) {
       int x[Op::NIN], y[Op::NOUT];
 #pragma unroll
       for (int k = 0; k < Op::NIN; k++) x[k] = (int)in[k][i];
@@ -313,21 +365,70 @@ __global__ __launch_bounds__(T) void rew_kernel(const RPtrs *__restrict__ tab, l
   }
 }
 
+// grid (blocks, entries, moduli)
+template <int OP, bool VEC4>
+__global__ __launch_bounds__(T) void rew_kernel(const RPtrs *__restrict__ tab, long long n,
+                                                const OzakiConsts *__restrict__ kc) {
+  rew_body<OP, VEC4>(tab[blockIdx.y], n, blockIdx.z, blockIdx.x, gridDim.x, kc);
+}
+
+// segments: gx blocks x (entries * moduli), aux = moduli
+template <int OP, bool VEC4>
+__global__ __launch_bounds__(T) void rew_multi_kernel(const char *__restrict__ base,
+                                                      const VSeg *__restrict__ segs, int nseg,
+                                                      const OzakiConsts *__restrict__ kc) {
+  const VSeg &S = segs[vseg_find(segs, nseg, blockIdx.x)];
+  const int l = blockIdx.x - S.blk0;
+  const int by = l / S.gx;
+  const RPtrs *tab = reinterpret_cast<const RPtrs *>(base + S.arg);
+  rew_body<OP, VEC4>(tab[by / S.aux], S.n, by % S.aux, l % S.gx, S.gx, kc);
+}
+
 template <int OP>
 hipError_t launch_rew_op(const RPtrs *tab, int count, long long n, int nmod, bool vec4,
                          const OzakiConsts *kc, hipStream_t s) {
-  const long long units = vec4 ? (n >> 2) : n;
-  long long blocks = (units + T - 1) / T;
-  long long cap = (long long)device_cu_count() * 8 / ((long long)count * nmod);
-  if (cap < 1) cap = 1;
-  if (blocks > cap) blocks = cap;
-  if (blocks < 1) blocks = 1;
-  dim3 grid((unsigned)blocks, (unsigned)count, (unsigned)nmod);
+  int gx, gy;
+  rew_grid(count, n, nmod, vec4, gx, gy);
+  dim3 grid((unsigned)gx, (unsigned)count, (unsigned)nmod);
   if (vec4)
     hipLaunchKernelGGL((rew_kernel<OP, true>), grid, dim3(T), 0, s, tab, n, kc);
   else
     hipLaunchKernelGGL((rew_kernel<OP, false>), grid, dim3(T), 0, s, tab, n, kc);
   return hipGetLastError();
+}
+
+template <int OP>
+hipError_t launch_rew_op_multi(const char *base, const VSeg *segs, int nseg, int blocks,
+                               bool vec4, const OzakiConsts *kc, hipStream_t s) {
+  if (vec4)
+    hipLaunchKernelGGL((rew_multi_kernel<OP, true>), dim3(blocks), dim3(T), 0, s, base, segs,
+                       nseg, kc);
+  else
+    hipLaunchKernelGGL((rew_multi_kernel<OP, false>), dim3(blocks), dim3(T), 0, s, base, segs,
+                       nseg, kc);
+  return hipGetLastError();
+}
+
+template <int K, bool F>
+hipError_t launch_crt_multi_k(int np, const char *base, const VSeg *segs, int nseg, int blocks,
+                              const OzakiConsts *kc, hipStream_t s) {
+  if (K != CRT_RECON || np <= 2)
+    hipLaunchKernelGGL((crt_multi_kernel<K, F, 2>), dim3(blocks), dim3(T), 0, s, base, segs, nseg,
+                       kc);
+  else if (np == 3)
+    hipLaunchKernelGGL((crt_multi_kernel<K, F, 3>), dim3(blocks), dim3(T), 0, s, base, segs, nseg,
+                       kc);
+  else
+    hipLaunchKernelGGL((crt_multi_kernel<K, F, 4>), dim3(blocks), dim3(T), 0, s, base, segs, nseg,
+                       kc);
+  return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_crt_multi_f(bool flag, int np, const char *base, const VSeg *segs, int nseg,
+                              int blocks, const OzakiConsts *kc, hipStream_t s) {
+  return flag ? launch_crt_multi_k<K, true>(np, base, segs, nseg, blocks, kc, s)
+              : launch_crt_multi_k<K, false>(np, base, segs, nseg, blocks, kc, s);
 }
 
 unsigned capped_blocks(long long units, long long per_grid) {
@@ -340,10 +441,9 @@ unsigned capped_blocks(long long units, long long per_grid) {
 
 template <int NP>
 hipError_t launch_recon(const CrtMats *M, const CrtMats &h, const OzakiConsts *kc, hipStream_t s) {
-  const bool fast = (h.m % 4 == 0) && (h.bs == 1 || h.bs == h.m);
-  const long long mm = (long long)h.m * h.m;
-  const int leaves = 1 << (2 * h.nrec);
-  dim3 grid(capped_blocks(fast ? mm / 4 : mm, leaves), leaves);
+  int gx, gy;
+  const bool fast = crt_grid(CRT_RECON, h, gx, gy);
+  dim3 grid((unsigned)gx, (unsigned)gy);
   if (fast)
     hipLaunchKernelGGL((crt_recon_kernel<NP, true>), grid, dim3(T), 0, s, M, kc);
   else
@@ -353,45 +453,121 @@ hipError_t launch_recon(const CrtMats *M, const CrtMats &h, const OzakiConsts *k
 
 }  // namespace
 
+void rew_grid(int count, long long n, int nmod, bool vec4, int &gx, int &gy) {
+  const long long units = vec4 ? (n >> 2) : n;
+  long long blocks = (units + T - 1) / T;
+  long long cap = (long long)device_cu_count() * 8 / ((long long)count * nmod);
+  if (cap < 1) cap = 1;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  gx = (int)blocks;
+  gy = count * nmod;
+}
+
+bool crt_grid(int k, const CrtMats &h, int &gx, int &gy) {
+  const int leaves = 1 << (2 * h.nrec);
+  const int nblk = 1 << h.nrec;
+  const bool plain = (h.bs == 1 || h.bs == h.m);
+  const long long mm = (long long)h.m * h.m;
+  gy = 1;
+  switch (k) {
+    case CRT_ROWPART:
+      gx = (h.n + T - 1) / T;
+      gy = nblk * h.chunks;
+      return plain;
+    case CRT_COLPART:
+      gx = (int)(((long long)h.n * nblk + 3) / 4);
+      return plain;
+    case CRT_FINALIZE:
+      gx = (h.n + T - 1) / T;
+      return false;
+    case CRT_SPLIT_A: {
+      const int tiles = (h.m + 63) / 64;
+      gx = tiles * tiles;
+      gy = leaves;
+      return plain && h.m % 64 == 0;
+    }
+    case CRT_SPLIT_B: {
+      const bool fb = plain && h.m % 16 == 0;
+      gx = (int)capped_blocks(fb ? mm / 16 : mm, leaves);
+      gy = leaves;
+      return fb;
+    }
+    case CRT_RECON: {
+      const bool fast = (h.m % 4 == 0) && plain;
+      gx = (int)capped_blocks(fast ? mm / 4 : mm, leaves);
+      gy = leaves;
+      return fast;
+    }
+  }
+  gx = 0;
+  return false;
+}
+
+hipError_t launch_crt_multi(int k, bool flag, int np, const char *base, const VSeg *segs,
+                            int nseg, int blocks, const OzakiConsts *kc, hipStream_t s) {
+  if (nseg <= 0 || blocks <= 0) return hipSuccess;
+  switch (k) {
+    case CRT_ROWPART: return launch_crt_multi_f<CRT_ROWPART>(flag, np, base, segs, nseg, blocks, kc, s);
+    case CRT_COLPART: return launch_crt_multi_f<CRT_COLPART>(flag, np, base, segs, nseg, blocks, kc, s);
+    case CRT_FINALIZE: return launch_crt_multi_f<CRT_FINALIZE>(false, np, base, segs, nseg, blocks, kc, s);
+    case CRT_SPLIT_A: return launch_crt_multi_f<CRT_SPLIT_A>(flag, np, base, segs, nseg, blocks, kc, s);
+    case CRT_SPLIT_B: return launch_crt_multi_f<CRT_SPLIT_B>(flag, np, base, segs, nseg, blocks, kc, s);
+    case CRT_RECON: return launch_crt_multi_f<CRT_RECON>(flag, np, base, segs, nseg, blocks, kc, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_residue_ew_multi(int op, const char *base, const VSeg *segs, int nseg,
+                                   int blocks, bool vec4, const OzakiConsts *kc, hipStream_t s) {
+  if (nseg <= 0 || blocks <= 0) return hipSuccess;
+  switch (op) {
+    case EW_BFS_PRE: return launch_rew_op_multi<EW_BFS_PRE>(base, segs, nseg, blocks, vec4, kc, s);
+    case EW_BFS_POST: return launch_rew_op_multi<EW_BFS_POST>(base, segs, nseg, blocks, vec4, kc, s);
+    case EW_DFS_PRE1: return launch_rew_op_multi<EW_DFS_PRE1>(base, segs, nseg, blocks, vec4, kc, s);
+    case EW_DFS_PRE2: return launch_rew_op_multi<EW_DFS_PRE2>(base, segs, nseg, blocks, vec4, kc, s);
+    case EW_DFS_PRE3: return launch_rew_op_multi<EW_DFS_PRE3>(base, segs, nseg, blocks, vec4, kc, s);
+    case EW_DFS_POST1: return launch_rew_op_multi<EW_DFS_POST1>(base, segs, nseg, blocks, vec4, kc, s);
+    case EW_DFS_POST2: return launch_rew_op_multi<EW_DFS_POST2>(base, segs, nseg, blocks, vec4, kc, s);
+    case EW_DFS_POST3: return launch_rew_op_multi<EW_DFS_POST3>(base, segs, nseg, blocks, vec4, kc, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_crt_step(int step, const CrtMats *M, const CrtMats &h, const OzakiConsts *kc,
                            hipStream_t s) {
   const int leaves = 1 << (2 * h.nrec);
-  const int nblk = 1 << h.nrec;
   if (leaves > 65535) return hipErrorInvalidValue;
+  int gx, gy;
+  bool f;
   switch (step) {
     case 0: {
-      const bool plain = (h.bs == 1 || h.bs == h.m);
-      dim3 g1((h.n + T - 1) / T, nblk * h.chunks);
-      if (plain)
-        hipLaunchKernelGGL((crt_rowpart_kernel<true>), g1, dim3(T), 0, s, M);
+      f = crt_grid(CRT_ROWPART, h, gx, gy);
+      if (f)
+        hipLaunchKernelGGL((crt_rowpart_kernel<true>), dim3(gx, gy), dim3(T), 0, s, M);
       else
-        hipLaunchKernelGGL((crt_rowpart_kernel<false>), g1, dim3(T), 0, s, M);
-      const long long waves = (long long)h.n * nblk;
-      dim3 g2((unsigned)((waves + 3) / 4));
-      if (plain)
-        hipLaunchKernelGGL((crt_colpart_kernel<true>), g2, dim3(T), 0, s, M);
+        hipLaunchKernelGGL((crt_rowpart_kernel<false>), dim3(gx, gy), dim3(T), 0, s, M);
+      f = crt_grid(CRT_COLPART, h, gx, gy);
+      if (f)
+        hipLaunchKernelGGL((crt_colpart_kernel<true>), dim3(gx), dim3(T), 0, s, M);
       else
-        hipLaunchKernelGGL((crt_colpart_kernel<false>), g2, dim3(T), 0, s, M);
-      dim3 g3((h.n + T - 1) / T);
-      hipLaunchKernelGGL(crt_finalize_kernel, g3, dim3(T), 0, s, M, kc);
+        hipLaunchKernelGGL((crt_colpart_kernel<false>), dim3(gx), dim3(T), 0, s, M);
+      crt_grid(CRT_FINALIZE, h, gx, gy);
+      hipLaunchKernelGGL(crt_finalize_kernel, dim3(gx), dim3(T), 0, s, M, kc);
       return hipGetLastError();
     }
     case 1: {
-      const bool col = (h.bs == 1 || h.bs == h.m);
-      const int tiles = (h.m + 63) / 64;
-      dim3 ga((unsigned)(tiles * tiles), leaves);
-      if (col && h.m % 64 == 0)
-        hipLaunchKernelGGL((crt_split_a_kernel<true>), ga, dim3(T), 0, s, M, kc);
+      f = crt_grid(CRT_SPLIT_A, h, gx, gy);
+      if (f)
+        hipLaunchKernelGGL((crt_split_a_kernel<true>), dim3(gx, gy), dim3(T), 0, s, M, kc);
       else
-        hipLaunchKernelGGL((crt_split_a_kernel<false>), ga, dim3(T), 0, s, M, kc);
+        hipLaunchKernelGGL((crt_split_a_kernel<false>), dim3(gx, gy), dim3(T), 0, s, M, kc);
       if (h.fused_b) return hipGetLastError();
-      const long long mm = (long long)h.m * h.m;
-      const bool fb = col && h.m % 16 == 0;
-      dim3 gb(capped_blocks(fb ? mm / 16 : mm, leaves), leaves);
-      if (fb)
-        hipLaunchKernelGGL((crt_split_b_kernel<true>), gb, dim3(T), 0, s, M, kc);
+      f = crt_grid(CRT_SPLIT_B, h, gx, gy);
+      if (f)
+        hipLaunchKernelGGL((crt_split_b_kernel<true>), dim3(gx, gy), dim3(T), 0, s, M, kc);
       else
-        hipLaunchKernelGGL((crt_split_b_kernel<false>), gb, dim3(T), 0, s, M, kc);
+        hipLaunchKernelGGL((crt_split_b_kernel<false>), dim3(gx, gy), dim3(T), 0, s, M, kc);
       return hipGetLastError();
     }
     case 2: {

@@ -34,7 +34,7 @@ namespace caps {
 bool PlanKey::operator==(const PlanKey &o) const {
   return A == o.A && B == o.B && C == o.C && memcmp(&desc, &o.desc, sizeof(desc)) == 0 &&
          pattern == o.pattern && mode == o.mode && moduli == o.moduli && cutoff == o.cutoff &&
-         fuse == o.fuse && batch == o.batch && stream == o.stream;
+         fuse == o.fuse && batch == o.batch && stream == o.stream && tile == o.tile;
 }
 
 namespace {
@@ -51,10 +51,14 @@ struct Builder {
   int cutoff = 0;  // layout levels below the Strassen recursion
   std::string pattern;
   bool fuse = true;      // fuse two leading 'b' levels when the sizes allow
+  int tile = 0;          // leaf-GEMM block tile (0 default, 1 smaller)
   bool residue = false;  // matrices are nmod INT8 planes
   int es = 8;            // bytes per element: 8 (FP64) or 1 (residue)
   int planes = 1;        // 1 or nmod
   bool broken = false;   // an internal layout invariant failed (plan unusable)
+  std::vector<Step> *out = nullptr;  // where steps go (default: plan.steps)
+
+  std::vector<Step> &steps() { return out ? *out : plan.steps; }
 
   size_t alloc(size_t bytes) {
     size_t off = top;
@@ -96,6 +100,7 @@ struct Builder {
             int levels = 0) {
     Step st{};
     st.levels = levels;
+    st.tile = tile;
     st.kind = kind;
     st.op = op;
     st.count = count;
@@ -103,7 +108,7 @@ struct Builder {
     st.bs = bs_;
     st.vec2 = vec;
     st.table = table;
-    plan.steps.push_back(st);
+    steps().push_back(st);
   }
 
   // one Winograd add step over the batch: ins[p][k], outs[p][k]
@@ -503,9 +508,9 @@ struct Builder {
       plan.crt_host.push_back(M);
       const size_t hi = plan.crt_host.size() - 1;
       push(STEP_CRT_SCALE, moduli, 1, n, d.bs, mt);
-      plan.steps.back().aux = hi;
+      steps().back().aux = hi;
       push(STEP_CRT_SPLIT, moduli, 1, n, d.bs, mt);
-      plan.steps.back().aux = hi;
+      steps().back().aux = hi;
       top_batch.push_back({RA, RB, RC});
       mats.push_back(mt);
     }
@@ -516,7 +521,7 @@ struct Builder {
     rec(top_batch, n, d.nrec, 0);
     for (size_t i = 0; i < items.size(); i++) {
       push(STEP_CRT_RECON, moduli, 1, n, d.bs, mats[i]);
-      plan.steps.back().aux = plan.crt_host.size() - items.size() + i;
+      steps().back().aux = plan.crt_host.size() - items.size() + i;
     }
   }
 };
@@ -530,7 +535,7 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
                        st.vec2 != 0, s);
     case STEP_GEMM_F64:
       return launch_dgemm_batched(reinterpret_cast<const GemmPtrs *>(tb + st.table), st.count,
-                                  (int)st.n, st.levels, s);
+                                  (int)st.n, st.levels, st.tile, s);
     case STEP_REORDER:
       return launch_leaf_reorder(reinterpret_cast<const ReorderPtrs *>(tb + st.table), st.count,
                                  (int)st.n, st.bs, st.op == 1, s);
@@ -539,7 +544,7 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
         return launch_residue_gemm_bf16(reinterpret_cast<const RGemm *>(tb + st.table), st.count,
                                         (int)st.n, st.levels, st.op, kc, s);
       return launch_residue_gemm(reinterpret_cast<const RGemm *>(tb + st.table), st.count,
-                                 (int)st.n, st.levels, st.op, kc, s);
+                                 (int)st.n, st.levels, st.op, kc, s, st.tile);
     case STEP_OZ_SCALE:
     case STEP_OZ_SPLIT:
     case STEP_OZ_RECON:
@@ -564,6 +569,22 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
       return launch_crt_step(st.kind - STEP_CRT_SCALE,
                              reinterpret_cast<const CrtMats *>(tb + st.table),
                              plan.crt_host[st.aux], kc, s);
+    case STEP_MULTI_EW:
+      return launch_ew_multi(st.op, tb, reinterpret_cast<const VSeg *>(tb + st.table), st.count,
+                             (int)st.n, st.vec2 != 0, s);
+    case STEP_MULTI_REW:
+      return launch_residue_ew_multi(st.op, tb, reinterpret_cast<const VSeg *>(tb + st.table),
+                                     st.count, (int)st.n, st.vec2 != 0, kc, s);
+    case STEP_MULTI_GEMM:
+      return launch_dgemm_multi(st.bs, tb, reinterpret_cast<const VSeg *>(tb + st.table),
+                                st.count, (int)st.n, s);
+    case STEP_MULTI_RGEMM:
+      return launch_residue_gemm_multi(tb, reinterpret_cast<const VSeg *>(tb + st.table),
+                                       st.count, (int)st.n, plan.oz_moduli, kc, s);
+    case STEP_MULTI_CRT:
+      return launch_crt_multi(st.op, st.vec2 != 0, ozaki_npieces(plan.oz_moduli), tb,
+                              reinterpret_cast<const VSeg *>(tb + st.table), st.count, (int)st.n,
+                              kc, s);
   }
   return hipErrorInvalidValue;
 }
@@ -585,6 +606,11 @@ int timer_of(const Step &st) {
     case STEP_BB_SPLIT_B: return CAPS_TIMER_SPLIT;
     case STEP_OZ_RECON:
     case STEP_CRT_RECON: return CAPS_TIMER_RECON;
+    case STEP_MULTI_EW:
+    case STEP_MULTI_REW: return CAPS_TIMER_ADD;
+    case STEP_MULTI_GEMM:
+    case STEP_MULTI_RGEMM: return CAPS_TIMER_MUL;
+    case STEP_MULTI_CRT: return st.op == CRT_RECON ? CAPS_TIMER_RECON : CAPS_TIMER_SPLIT;
   }
   return CAPS_TIMER_COMM;
 }
@@ -593,7 +619,7 @@ int timer_of(const Step &st) {
 
 int build_plan(Plan &plan, const double *A, const double *B, double *C, const caps_desc &d,
                const std::string &pattern, int mode, int moduli, int cutoff, void *work_base,
-               bool fuse) {
+               bool fuse, int tile) {
   plan.steps.clear();
   plan.host_tables.clear();
   plan.additions = plan.multiplications = plan.leaves = 0;
@@ -606,6 +632,7 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
   b.moduli = moduli;
   b.cutoff = cutoff;
   b.fuse = fuse;
+  b.tile = tile;
   b.pattern = pattern;
   b.residue = residue_mode(mode);
   b.es = b.residue ? 1 : 8;
@@ -623,8 +650,179 @@ int build_plan(Plan &plan, const double *A, const double *B, double *C, const ca
   return b.broken ? CAPS_EINVAL : CAPS_OK;
 }
 
+namespace {
+
+// Which steps of different problems can share one segmented launch: equal
+// keys merge; kind < 0 never merges.
+struct MergeKey {
+  int kind = -1, a = 0, b = 0;
+  bool operator==(const MergeKey &o) const { return kind == o.kind && a == o.a && b == o.b; }
+};
+
+constexpr int MERGE_CRT_SCALE = 100, MERGE_CRT_SPLIT = 101, MERGE_CRT_RECON = 102;
+
+MergeKey merge_key(const Plan &plan, const Step &st) {
+  MergeKey k;
+  int gx, gy;
+  switch (st.kind) {
+    case STEP_EW: k = {STEP_MULTI_EW, st.op, st.vec2}; break;
+    case STEP_REW: k = {STEP_MULTI_REW, st.op, st.vec2}; break;
+    case STEP_GEMM_F64:
+      k = {STEP_MULTI_GEMM, dgemm_block_tile((int)st.n, st.levels, st.tile), 0};
+      break;
+    case STEP_OZ_GEMM: {
+      // INT8 products that would take the 128 x 128 kernel alone; those with
+      // 256-tile leaves keep their own launch
+      const int m = (int)st.n, m0 = m >> st.levels;
+      const bool big = st.tile == 0 && m0 % 256 == 0 && (long long)m * m < (1LL << 31);
+      if (st.bs == 0 && !big && residue_gemm_check(m, st.levels) >= 0)
+        k = {STEP_MULTI_RGEMM, 0, 0};
+      break;
+    }
+    case STEP_CRT_SCALE:
+      k = {STEP_MULTI_CRT, MERGE_CRT_SCALE, crt_grid(CRT_ROWPART, plan.crt_host[st.aux], gx, gy)};
+      break;
+    case STEP_CRT_SPLIT: {
+      const CrtMats &h = plan.crt_host[st.aux];
+      const int fa = crt_grid(CRT_SPLIT_A, h, gx, gy), fb = crt_grid(CRT_SPLIT_B, h, gx, gy);
+      k = {STEP_MULTI_CRT, MERGE_CRT_SPLIT, fa | (fb << 1) | (h.fused_b << 2)};
+      break;
+    }
+    case STEP_CRT_RECON:
+      k = {STEP_MULTI_CRT, MERGE_CRT_RECON, crt_grid(CRT_RECON, plan.crt_host[st.aux], gx, gy)};
+      break;
+  }
+  return k;
+}
+
+size_t push_segs(Plan &plan, const std::vector<VSeg> &segs) {
+  auto &t = plan.host_tables;
+  const size_t off = (t.size() + 63) / 64 * 64;
+  t.resize(off + segs.size() * sizeof(VSeg));
+  memcpy(t.data() + off, segs.data(), segs.size() * sizeof(VSeg));
+  return off;
+}
+
+Step multi_step(int kind, int op, int flag, const std::vector<VSeg> &segs, long long total,
+                size_t table) {
+  Step st{};
+  st.kind = kind;
+  st.op = op;
+  st.vec2 = flag;
+  st.bs = flag;
+  st.count = (int)segs.size();
+  st.n = total;
+  st.table = table;
+  return st;
+}
+
+// one CRT kernel over the items of `heads`
+void emit_crt(Plan &plan, const std::vector<const Step *> &heads, int k, bool flag) {
+  std::vector<VSeg> segs;
+  int blk = 0;
+  for (const Step *st : heads) {
+    VSeg v{};
+    v.arg = (long long)st->table;
+    v.blk0 = blk;
+    crt_grid(k, plan.crt_host[st->aux], v.gx, v.gy);
+    blk += v.gx * v.gy;
+    segs.push_back(v);
+  }
+  const size_t off = push_segs(plan, segs);
+  plan.steps.push_back(multi_step(STEP_MULTI_CRT, k, flag, segs, blk, off));
+}
+
+// the merged launch(es) for steps with key k
+void emit_merged(Plan &plan, const MergeKey &k, const std::vector<const Step *> &heads,
+                 int moduli) {
+  if (k.kind == STEP_MULTI_CRT) {
+    if (k.a == MERGE_CRT_SCALE) {
+      emit_crt(plan, heads, CRT_ROWPART, k.b);
+      emit_crt(plan, heads, CRT_COLPART, k.b);
+      emit_crt(plan, heads, CRT_FINALIZE, false);
+    } else if (k.a == MERGE_CRT_SPLIT) {
+      emit_crt(plan, heads, CRT_SPLIT_A, k.b & 1);
+      if (!(k.b & 4)) emit_crt(plan, heads, CRT_SPLIT_B, (k.b >> 1) & 1);
+    } else {
+      emit_crt(plan, heads, CRT_RECON, k.b);
+    }
+    return;
+  }
+  std::vector<VSeg> segs;
+  long long total = 0;
+  for (const Step *st : heads) {
+    VSeg v{};
+    v.arg = (long long)st->table;
+    v.n = st->n;
+    v.blk0 = (int)total;
+    if (k.kind == STEP_MULTI_EW) {
+      ew_grid(st->count, st->n, st->vec2 != 0, v.gx, v.gy);
+    } else if (k.kind == STEP_MULTI_REW) {
+      rew_grid(st->count, st->n, moduli, st->vec2 != 0, v.gx, v.gy);
+      v.aux = moduli;
+    } else if (k.kind == STEP_MULTI_GEMM) {
+      const int m = (int)st->n, bt = k.a;
+      v.gx = (m + bt - 1) / bt;
+      v.gy = st->count;
+      v.aux = st->levels | (((m >> st->levels) % bt != 0) << 8);
+    } else {  // STEP_MULTI_RGEMM
+      const int m = (int)st->n;
+      v.gx = (m + 127) / 128;
+      v.gy = st->count * moduli;
+      v.aux = st->levels | (residue_gemm_check(m, st->levels) << 8);
+    }
+    // elementwise segments count blocks (gx x gy), GEMM segments tiles
+    // (gx x gx per problem, gy problems)
+    const bool gemm = k.kind == STEP_MULTI_GEMM || k.kind == STEP_MULTI_RGEMM;
+    total += gemm ? (long long)v.gx * v.gx * v.gy : (long long)v.gx * v.gy;
+    segs.push_back(v);
+  }
+  const size_t off = push_segs(plan, segs);
+  // op: the Winograd op of an add step; flag: 16/4-byte accesses (adds) or
+  // the FP64 block tile
+  plan.steps.push_back(multi_step(k.kind, k.a, k.kind == STEP_MULTI_GEMM ? k.a : k.b, segs,
+                                  total, off));
+}
+
+// Interleave the groups' step lists: repeatedly take the key shared by the
+// most groups at their current step (ties: the earliest group), launch those
+// steps together, advance those groups.  Each group's own order is kept; the
+// groups are independent (own operands and workspace), so any interleaving
+// is valid.
+void merge_groups(Plan &plan, const std::vector<std::vector<Step>> &gsteps, int moduli) {
+  const size_t G = gsteps.size();
+  std::vector<size_t> pos(G, 0);
+  for (;;) {
+    std::vector<MergeKey> keys(G);
+    int best = -1, best_n = 0;
+    for (size_t g = 0; g < G; g++) {
+      if (pos[g] >= gsteps[g].size()) continue;
+      keys[g] = merge_key(plan, gsteps[g][pos[g]]);
+      int n = 0;
+      if (keys[g].kind >= 0)
+        for (size_t h = 0; h < G; h++)
+          if (pos[h] < gsteps[h].size() && merge_key(plan, gsteps[h][pos[h]]) == keys[g]) n++;
+      if (best < 0 || n > best_n) {
+        best = (int)g;
+        best_n = n;
+      }
+    }
+    if (best < 0) return;
+    if (keys[best].kind < 0 || best_n < 2) {
+      plan.steps.push_back(gsteps[best][pos[best]++]);
+      continue;
+    }
+    std::vector<const Step *> heads;
+    for (size_t g = 0; g < G; g++)
+      if (pos[g] < gsteps[g].size() && keys[g] == keys[best]) heads.push_back(&gsteps[g][pos[g]++]);
+    emit_merged(plan, keys[best], heads, moduli);
+  }
+}
+
+}  // namespace
+
 int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, int moduli,
-                     void *work_base) {
+                     void *work_base, int tile, size_t merge_budget) {
   plan.steps.clear();
   plan.host_tables.clear();
   plan.additions = plan.multiplications = plan.leaves = 0;
@@ -644,9 +842,14 @@ int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, 
       }
     if (!placed) groups.push_back({&it});
   }
-  size_t peak = 0;
-  for (auto &g : groups) {
-    Builder b{plan, reinterpret_cast<char *>(work_base)};
+  // Merged schedule: every group gets a workspace region of its own and the
+  // same step of different groups runs as one segmented launch.  When the
+  // regions together exceed merge_budget, the groups run one after another
+  // in one shared region instead.
+  auto build_group = [&](Plan &into, const std::vector<const BatchItem *> &g, char *base,
+                         std::vector<Step> *out, size_t &peak) {
+    Builder b{into, base};
+    b.out = out;
     b.bs = g[0]->desc.bs;
     b.mode = mode;
     b.moduli = moduli;
@@ -656,6 +859,7 @@ int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, 
     b.es = b.residue ? 1 : 8;
     b.planes = b.residue ? moduli : 1;
     b.fuse = true;
+    b.tile = tile;
     if (b.residue) {
       b.crt_group(g);
     } else {
@@ -666,10 +870,40 @@ int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, 
                          {reinterpret_cast<char *>(it->C), 0}});
       b.rec(batch, g[0]->desc.lda, g[0]->desc.nrec, 0);
     }
-    peak = std::max(peak, b.peak);
-    if (b.broken) return CAPS_EINVAL;
+    peak = (b.peak + ALIGN - 1) / ALIGN * ALIGN;
+    return !b.broken;
+  };
+  // each group's workspace, from a throwaway build with the same items
+  std::vector<size_t> peaks(groups.size());
+  size_t sum = 0, mx = 0;
+  for (size_t g = 0; g < groups.size(); g++) {
+    Plan scratch;
+    std::vector<Step> tmp;
+    if (!build_group(scratch, groups[g], nullptr, &tmp, peaks[g])) return CAPS_EINVAL;
+    sum += peaks[g];
+    mx = std::max(mx, peaks[g]);
   }
-  plan.work_bytes = peak;
+  const bool merge = groups.size() > 1 && sum <= merge_budget;
+  char *base = reinterpret_cast<char *>(work_base);
+  if (merge) {
+    std::vector<std::vector<Step>> gsteps(groups.size());
+    size_t off = 0;
+    for (size_t g = 0; g < groups.size(); g++) {
+      size_t pk = 0;
+      if (!build_group(plan, groups[g], base ? base + off : nullptr, &gsteps[g], pk) ||
+          pk != peaks[g])
+        return CAPS_EINVAL;
+      off += peaks[g];
+    }
+    merge_groups(plan, gsteps, moduli);
+    plan.work_bytes = sum;
+  } else {
+    for (auto &g : groups) {
+      size_t pk = 0;
+      if (!build_group(plan, g, base, nullptr, pk)) return CAPS_EINVAL;
+    }
+    plan.work_bytes = mx;
+  }
   plan.exec_pattern = "BATCH,";
   return CAPS_OK;
 }

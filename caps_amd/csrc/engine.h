@@ -45,6 +45,14 @@ enum StepKind : int {
   STEP_BB_PRE = 11,    // two 'b' levels of pre-additions in one pass (op = residue)
   STEP_BB_POST = 12,   // two 'b' levels of combinations (residue: + reconstruction)
   STEP_BB_SPLIT_B = 13,  // residue mode: B's split fused with its two top 'b' levels
+  // segmented launches (kernels.h VSeg) merging one step of several
+  // problems of an uneven batch: table = segment array, count = segments,
+  // n = flat blocks (or tiles) of the launch
+  STEP_MULTI_EW = 14,    // FP64 Winograd adds, op = EwOp, vec2 = 16-byte accesses
+  STEP_MULTI_REW = 15,   // residue Winograd adds, op = EwOp, vec2 = 4-byte accesses
+  STEP_MULTI_GEMM = 16,  // FP64 leaf GEMMs, bs = block tile
+  STEP_MULTI_RGEMM = 17, // INT8 residue leaf GEMMs (128 x 128 tiles)
+  STEP_MULTI_CRT = 18,   // residue scale / split / reconstruct kernel op (CrtKernel), vec2 = flag
 };
 
 struct Step {
@@ -55,6 +63,7 @@ struct Step {
   int bs;
   int vec2;
   int levels;     // leaf GEMMs: quadrant levels inside the operands (cutoff)
+  int tile;       // leaf GEMMs: 0 default block tile, 1 the smaller one
   size_t table;   // byte offset of this step's pointer table
   size_t aux;     // byte offset of an auxiliary workspace region
 };
@@ -70,6 +79,7 @@ struct PlanKey {
   bool fuse;
   std::string batch;  // batched plans: every item's pointers, descriptor and pattern
   const void *stream;
+  int tile;           // leaf-GEMM block tile: 0 default, 1 the smaller one
   bool operator==(const PlanKey &o) const;
 };
 
@@ -107,13 +117,17 @@ struct BatchItem {
 // work_base = nullptr computes sizes only.
 int build_plan(Plan &plan, const double *A, const double *B, double *C,
                const caps_desc &d, const std::string &pattern, int mode, int moduli,
-               int cutoff, void *work_base, bool fuse = true);
+               int cutoff, void *work_base, bool fuse = true, int tile = 0);
 
 // Build one plan for independent multiplies of possibly different shapes:
-// items with the same (descriptor, pattern) share batched steps; groups run
-// one after another and reuse the same workspace.
+// items with the same (descriptor, pattern, cutoff) form a group that shares
+// batched steps.  If the groups' workspaces together fit merge_budget bytes,
+// each group gets its own region and the same step of different groups runs
+// as one segmented launch (one leaf-GEMM launch for all shapes, one launch
+// per Winograd phase, one per scale / split / reconstruct kernel); otherwise
+// the groups run one after another in one shared region.
 int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, int moduli,
-                     void *work_base);
+                     void *work_base, int tile = 0, size_t merge_budget = 0);
 
 // Enqueue every step of the plan on `s` (no graph).
 hipError_t run_steps(const Plan &plan, hipStream_t s);

@@ -62,9 +62,47 @@ struct GemmPtrs {
 // quadtrees of `levels` quadrant levels over column-major leaves of size
 // m >> levels (levels = 0: plain column-major; levels > 0 needs leaves that
 // are multiples of 128).  One persistent launch over all tiles of all problems.
-hipError_t launch_dgemm_batched(const GemmPtrs *d_tab, int count, int m, int levels,
+// tile: 0 = 128 x 128 block tiles, 1 = 64 x 64.
+hipError_t launch_dgemm_batched(const GemmPtrs *d_tab, int count, int m, int levels, int tile,
                                 hipStream_t s);
 
 int device_cu_count();
+
+// ---------------------------------------------------------------- segmented launches
+// One launch over the same step of several independent problems (the uneven
+// batch): segment s owns the flat blocks [blk0, blk0 + gx * gy) and runs them
+// as a gx x gy grid over its own argument, which sits `arg` bytes into the
+// plan's device tables (segment tables are built before those tables have a
+// device address).  n is the segment's element count where the step has one.
+struct VSeg {
+  long long arg;
+  long long n;
+  int blk0, gx, gy, aux;
+};
+
+// segment of flat block b (segments sorted by blk0, the first at 0)
+__device__ __forceinline__ int vseg_find(const VSeg *__restrict__ segs, int nseg, int b) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// FP64 leaf GEMMs over segments of different sizes, one persistent launch:
+// segment args are GemmPtrs tables, n = order, gx = tiles per dimension, gy =
+// problems, blk0 = first global tile, aux = quadtree levels | edge checks <<
+// 8.  bt: block tile (dgemm_block_tile).
+int dgemm_block_tile(int m, int levels, int tile);
+hipError_t launch_dgemm_multi(int bt, const char *d_tables, const VSeg *d_segs, int nseg,
+                              int total_tiles, hipStream_t s);
+
+// Winograd add step `op` over segments whose args are EwPtrs tables (gy =
+// entries, n = elements per quadrant); vec2: every entry 16-byte aligned
+hipError_t launch_ew_multi(int op, const char *d_tables, const VSeg *d_segs, int nseg,
+                           int blocks, bool vec2, hipStream_t s);
+// the local grid launch_ew gives one step of `count` entries of n elements
+void ew_grid(int count, long long n, bool vec2, int &gx, int &gy);
 
 }  // namespace caps

@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kernels.h"
+
 namespace caps {
 
 constexpr int OZ_MAX_MODULI = 16;  // M <= 2^128: X fits 128 bits
@@ -90,7 +92,16 @@ hipError_t launch_ozaki_step(int step, const OzLeaf *d_tab, int count, int m, in
 // the same products on BF16 MFMA with FP32 accumulation (bit-identical results)
 hipError_t launch_residue_gemm_bf16(const RGemm *d_tab, int count, int m, int levels, int nmod,
                                     const OzakiConsts *d_consts, hipStream_t s);
+// The 128 x 128 residue products over segments of different sizes in one
+// persistent launch (kernels.h VSeg: args are RGemm tables, n = order, gx =
+// tiles per dimension, gy = leaves x moduli, aux = quadtree levels | check
+// mode << 8, the mode residue_gemm_check gives; -1: not supported).
+int residue_gemm_check(int m, int levels);
+hipError_t launch_residue_gemm_multi(const char *d_tables, const VSeg *d_segs, int nseg,
+                                     int total_tiles, int nmod, const OzakiConsts *d_consts,
+                                     hipStream_t s);
+// tile: 0 = 256 x 256 block tiles where the leaves allow, 1 = 128 x 128
 hipError_t launch_residue_gemm(const RGemm *d_tab, int count, int m, int levels, int nmod,
-                               const OzakiConsts *d_consts, hipStream_t s);
+                               const OzakiConsts *d_consts, hipStream_t s, int tile = 0);
 
 }  // namespace caps

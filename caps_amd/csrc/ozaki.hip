@@ -254,10 +254,102 @@ __device__ __forceinline__ void lstore_i8(int8_t *As, int8_t *Bs, int tid, const
   }
 }
 
-// problem p = leaf * nmod + l: C_l = A_l * B_l (mod m_l)
-// Operands are m x m quadtrees of d levels over leaves of size m0 = m >> d in
-// CAPS quadrant order: A leaves row-major, B and C leaves column-major (d = 0:
-// plain matrices).  The fast path needs m0 % 128 == 0.
+// One 128 x 128 tile (ti, tj) of problem p = leaf * nmod + l: C_l = A_l *
+// B_l (mod m_l).  Operands are m x m quadtrees of d levels over leaves of size
+// m0 = m >> d in CAPS quadrant order: A leaves row-major, B and C leaves
+// column-major (d = 0: plain matrices).  The fast path needs m0 % 128 == 0.
+template <int CHECK>
+__device__ __forceinline__ void oz_tile(const RGemm *__restrict__ tab, int m, int nmod,
+                                        const OzakiConsts *__restrict__ kc, int m0, int d,
+                                        int prob, int ti, int tj, int8_t (*As)[IBM * ISTR],
+                                        int8_t (*Bs)[IBN * ISTR]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l16 = lane & 15, lk = lane >> 4;
+  const int nk = (m + IBK - 1) / IBK;
+  const int leaf = prob / nmod, l = prob - leaf * nmod;
+  const int8_t *__restrict__ Ap = tab[leaf].a + l * tab[leaf].sa;  // row-major
+  const int8_t *__restrict__ Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major
+  int8_t *__restrict__ Cp = tab[leaf].c + l * tab[leaf].sc;
+  const int mi = kc->m[l];
+  const float mf = (float)mi, invf = 1.0f / mf;
+  const int row0 = ti * IBM, col0 = tj * IBN;
+
+  // small K: the sums start at 128 for the one-pass reduction res_byte128
+  const bool small_k = m <= 16384;
+  const int bias = small_k ? 128 : 0;
+  v4i acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[r][c] = v4i{bias, bias, bias, bias};
+
+  v4u ga[2], gb[2];
+  const long long mm0 = (long long)m0 * m0;
+  auto a_at = [&](int k0) {  // A slice origin (row0, k0), row-major leaf
+    return Ap + quad_leaf(row0 / m0, k0 / m0, d) * mm0 + (long long)(row0 % m0) * m0 + (k0 % m0);
+  };
+  auto b_at = [&](int k0) {  // B slice origin (k0, col0), column-major leaf
+    return Bp + quad_leaf(k0 / m0, col0 / m0, d) * mm0 + (k0 % m0) + (long long)(col0 % m0) * m0;
+  };
+  gload_i8<CHECK>(a_at(0), b_at(0), m0, m, row0, col0, 0, tid, ga, gb);
+  lstore_i8(As[0], Bs[0], tid, ga, gb);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk)
+      gload_i8<CHECK>(a_at((kt + 1) * IBK), b_at((kt + 1) * IBK), m0, m, row0, col0,
+                      (kt + 1) * IBK, tid, ga, gb);
+    v4i a[4], b[4];
+    // lane group lk supplies k = 16*lk .. 16*lk+15 for both operands
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      a[r] = *reinterpret_cast<const v4i *>(&As[buf][(wm * 64 + r * 16 + l16) * ISTR + 16 * lk]);
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      b[c] = *reinterpret_cast<const v4i *>(&Bs[buf][(wn * 64 + c * 16 + l16) * ISTR + 16 * lk]);
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], b[c], acc[r][c], 0, 0, 0);
+    if (kt + 1 < nk) lstore_i8(As[buf ^ 1], Bs[buf ^ 1], tid, ga, gb);
+    __syncthreads();
+  }
+
+  // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int row = row0 + wm * 64 + r * 16 + 4 * lk;
+      const int col = col0 + wn * 64 + c * 16 + l16;
+      int8_t *dst = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0 + (long long)(col % m0) * m0 +
+                    (row % m0);
+      unsigned v = 0;
+      if (small_k) {
+        const int c14m = kc->c14[l] - 16384;
+        const float cl = 0.5f - 128.0f * invf;
+#pragma unroll
+        for (int q = 0; q < 4; q++) v = res_byte128(acc[r][c][q], c14m, mf, invf, cl, q, v);
+        v ^= 0x80808080u;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
+      }
+      if (CHECK == 0) {
+        *(gu32 *)dst = v;
+      } else if (CHECK >= 2) {
+        // m % 16 == 0: the 4 rows of the word are all inside or all outside
+        if (col < m && row < m) *(gu32 *)dst = v;
+      } else if (col < m) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (row + q < m) dst[q] = (int8_t)(v >> (8 * q));
+      }
+    }
+}
+
 template <int CHECK>
 __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict__ tab, int m,
                                                         int nmod,
@@ -265,98 +357,41 @@ __global__ __launch_bounds__(T256) void oz_igemm_kernel(const RGemm *__restrict_
                                                         TileMap map, int m0, int d) {
   __shared__ __attribute__((aligned(16))) int8_t As[2][IBM * ISTR];
   __shared__ __attribute__((aligned(16))) int8_t Bs[2][IBN * ISTR];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int l16 = lane & 15, lk = lane >> 4;
   const int G = gridDim.x;
-  const int nk = (m + IBK - 1) / IBK;
-
   for (int round = 0; round * G < map.total; round++) {
     const int tile = round * G + xcd_tile(blockIdx.x, G);
     if (tile >= map.total) break;
     int prob, ti, tj;
     tile_coords(map, tile, prob, ti, tj);
-    const int leaf = prob / nmod, l = prob - leaf * nmod;
-    const int8_t *__restrict__ Ap = tab[leaf].a + l * tab[leaf].sa;  // row-major
-    const int8_t *__restrict__ Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major
-    int8_t *__restrict__ Cp = tab[leaf].c + l * tab[leaf].sc;
-    const int mi = kc->m[l];
-    const float mf = (float)mi, invf = 1.0f / mf;
-    const int row0 = ti * IBM, col0 = tj * IBN;
+    oz_tile<CHECK>(tab, m, nmod, kc, m0, d, prob, ti, tj, As, Bs);
+  }
+}
 
-    // small K: the sums start at 128 for the one-pass reduction res_byte128
-    const bool small_k = m <= 16384;
-    const int bias = small_k ? 128 : 0;
-    v4i acc[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-      for (int c = 0; c < 4; c++) acc[r][c] = v4i{bias, bias, bias, bias};
-
-    v4u ga[2], gb[2];
-    const long long mm0 = (long long)m0 * m0;
-    auto a_at = [&](int k0) {  // A slice origin (row0, k0), row-major leaf
-      return Ap + quad_leaf(row0 / m0, k0 / m0, d) * mm0 + (long long)(row0 % m0) * m0 + (k0 % m0);
-    };
-    auto b_at = [&](int k0) {  // B slice origin (k0, col0), column-major leaf
-      return Bp + quad_leaf(k0 / m0, col0 / m0, d) * mm0 + (k0 % m0) + (long long)(col0 % m0) * m0;
-    };
-    gload_i8<CHECK>(a_at(0), b_at(0), m0, m, row0, col0, 0, tid, ga, gb);
-    lstore_i8(As[0], Bs[0], tid, ga, gb);
-    __syncthreads();
-    for (int kt = 0; kt < nk; kt++) {
-      const int buf = kt & 1;
-      if (kt + 1 < nk)
-        gload_i8<CHECK>(a_at((kt + 1) * IBK), b_at((kt + 1) * IBK), m0, m, row0, col0,
-                        (kt + 1) * IBK, tid, ga, gb);
-      v4i a[4], b[4];
-      // lane group lk supplies k = 16*lk .. 16*lk+15 for both operands
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        a[r] = *reinterpret_cast<const v4i *>(&As[buf][(wm * 64 + r * 16 + l16) * ISTR + 16 * lk]);
-#pragma unroll
-      for (int c = 0; c < 4; c++)
-        b[c] = *reinterpret_cast<const v4i *>(&Bs[buf][(wn * 64 + c * 16 + l16) * ISTR + 16 * lk]);
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-          acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], b[c], acc[r][c], 0, 0, 0);
-      if (kt + 1 < nk) lstore_i8(As[buf ^ 1], Bs[buf ^ 1], tid, ga, gb);
-      __syncthreads();
+// The residue products of several sizes in one persistent launch: segment
+// args are RGemm tables, n = order, gx = tiles per dimension, gy = problems
+// (leaves x moduli), aux = quadtree levels | check mode << 8.
+__global__ __launch_bounds__(T256) void oz_igemm_multi_kernel(const char *__restrict__ base,
+                                                              const VSeg *__restrict__ segs,
+                                                              int nseg, int total, int nmod,
+                                                              const OzakiConsts *__restrict__ kc) {
+  __shared__ __attribute__((aligned(16))) int8_t As[2][IBM * ISTR];
+  __shared__ __attribute__((aligned(16))) int8_t Bs[2][IBN * ISTR];
+  const int G = gridDim.x;
+  for (int round = 0; round * G < total; round++) {
+    const int tile = round * G + xcd_tile(blockIdx.x, G);
+    if (tile >= total) break;
+    const VSeg &S = segs[vseg_find(segs, nseg, tile)];
+    const int m = (int)S.n, d = S.aux & 255, check = S.aux >> 8;
+    const int local = tile - S.blk0, per = S.gx * S.gx;
+    const int prob = local / per, r = local - prob * per;
+    const int ti = r % S.gx, tj = r / S.gx;
+    const RGemm *tab = reinterpret_cast<const RGemm *>(base + S.arg);
+    switch (check) {
+      case 0: oz_tile<0>(tab, m, nmod, kc, m >> d, d, prob, ti, tj, As, Bs); break;
+      case 2: oz_tile<2>(tab, m, nmod, kc, m >> d, d, prob, ti, tj, As, Bs); break;
+      case 3: oz_tile<3>(tab, m, nmod, kc, m >> d, d, prob, ti, tj, As, Bs); break;
+      default: oz_tile<1>(tab, m, nmod, kc, m >> d, d, prob, ti, tj, As, Bs); break;
     }
-
-    // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const int row = row0 + wm * 64 + r * 16 + 4 * lk;
-        const int col = col0 + wn * 64 + c * 16 + l16;
-        int8_t *dst = Cp + quad_leaf(row0 / m0, col0 / m0, d) * mm0 +
-                      (long long)(col % m0) * m0 + (row % m0);
-        unsigned v = 0;
-        if (small_k) {
-          const int c14m = kc->c14[l] - 16384;
-          const float cl = 0.5f - 128.0f * invf;
-#pragma unroll
-          for (int q = 0; q < 4; q++) v = res_byte128(acc[r][c][q], c14m, mf, invf, cl, q, v);
-          v ^= 0x80808080u;
-        } else {
-#pragma unroll
-          for (int q = 0; q < 4; q++) v |= (unsigned)(mod_i32(acc[r][c][q], mi, invf) & 0xff) << (8 * q);
-        }
-        if (CHECK == 0) {
-          *(gu32 *)dst = v;
-        } else if (CHECK >= 2) {
-          // m % 16 == 0: the 4 rows of the word are all inside or all outside
-          if (col < m && row < m) *(gu32 *)dst = v;
-        } else if (col < m) {
-#pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (row + q < m) dst[q] = (int8_t)(v >> (8 * q));
-        }
-      }
   }
 }
 
@@ -435,19 +470,31 @@ __device__ __forceinline__ void stage_barrier() {
 struct BigTile {
   const int8_t *Ap, *Bp;
   int8_t *Cp;
+  __amdgpu_buffer_rsrc_t ra, rb;  // buffer descriptors of the operand planes
   int mi, c14, row0, col0, ti, tj;
   float mf, invf;
 };
 
+// a uniform pointer kept in scalar registers (buffer descriptors built from a
+// pointer the compiler thinks may vary per lane are used through per-lane
+// loops)
+template <typename P>
+__device__ __forceinline__ P *uniform_ptr(P *p) {
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return reinterpret_cast<P *>(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ BigTile big_tile(const RGemm *__restrict__ tab, const TileMap &map,
                                             int tile, int nmod,
-                                            const OzakiConsts *__restrict__ kc) {
+                                            const OzakiConsts *__restrict__ kc, long long opb) {
   BigTile t;
   int prob, ti, tj;
   tile_coords(map, tile, prob, ti, tj);
   const int leaf = prob / nmod, l = prob - leaf * nmod;
-  t.Ap = tab[leaf].a + l * tab[leaf].sa;  // row-major leaves
-  t.Bp = tab[leaf].b + l * tab[leaf].sb;  // column-major leaves
+  t.Ap = uniform_ptr(tab[leaf].a + l * tab[leaf].sa);  // row-major leaves
+  t.Bp = uniform_ptr(tab[leaf].b + l * tab[leaf].sb);  // column-major leaves
   t.Cp = tab[leaf].c + l * tab[leaf].sc;
   t.mi = kc->m[l];
   t.mf = (float)t.mi;
@@ -457,6 +504,8 @@ __device__ __forceinline__ BigTile big_tile(const RGemm *__restrict__ tab, const
   t.col0 = tj * JBM;
   t.ti = ti;
   t.tj = tj;
+  t.ra = plane_rsrc(t.Ap, opb);
+  t.rb = plane_rsrc(t.Bp, opb);
   return t;
 }
 
@@ -467,7 +516,7 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
   __shared__ __attribute__((aligned(1024))) int8_t A1[JSTAGE];
   __shared__ __attribute__((aligned(1024))) int8_t B0[JSTAGE];
   __shared__ __attribute__((aligned(1024))) int8_t B1[JSTAGE];
-  __shared__ __attribute__((aligned(256))) unsigned PF[8][64];  // prefetch sink
+  __shared__ __attribute__((aligned(256))) unsigned PF[2][64];  // prefetch sink
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves: rows wm * 128, columns wn * 64
   const int l16 = lane & 15, lk = lane >> 4;
@@ -489,27 +538,30 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
 
   int tile = xcd_tile(blockIdx.x, G);
   if (tile >= map.total) return;
-  BigTile t = big_tile(tab, map, tile, nmod, kc);
-  // offsets of row `row` / column `col` at k in the quadtree operands
+  BigTile t = big_tile(tab, map, tile, nmod, kc, opb);
+  // offsets of row `row` / column `col` at k in the quadtree operands; plain
+  // operands (d == 0) need no division by the leaf size
   auto a_off = [&](int row, int k) -> long long {
-    return quad_leaf(row / m0, k / m0, d) * mm0 + (long long)(row % m0) * m0 + (k % m0);
+    return d == 0 ? (long long)row * m0 + k
+                  : quad_leaf(row / m0, k / m0, d) * mm0 + (long long)(row % m0) * m0 + (k % m0);
   };
   auto b_off = [&](int col, int k) -> long long {
-    return quad_leaf(k / m0, col / m0, d) * mm0 + (long long)(col % m0) * m0 + (k % m0);
+    return d == 0 ? (long long)col * m0 + k
+                  : quad_leaf(k / m0, col / m0, d) * mm0 + (long long)(col % m0) * m0 + (k % m0);
   };
   // waves 0 and 1: this block's 1/32 share of the patch's A / B slab at k
   auto prefetch = [&](const BigTile &c, int k) {
     const int w = (c.ti & 7) + 8 * (c.tj & 3);
     const int pr0 = (c.ti & ~7) * JBM, pc0 = (c.tj & ~3) * JBM;
-    const int8_t *p = wave == 0 ? c.Ap + a_off(pr0 + w * 64 + lane, k)
-                                : c.Bp + b_off(pc0 + w * 32 + (lane & 31), k);
-    const int8_t *base = wave == 0 ? c.Ap : c.Bp;
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(plane_rsrc(base, opb), (lds_ptr)&PF[wave][0], 4,
-                                             (int)(p - base), 0, 0, 0);
+    if (wave == 0)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(c.ra, (lds_ptr)&PF[0][0], 4,
+                                               (int)a_off(pr0 + w * 64 + lane, k), 0, 0, 0);
+    else
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(c.rb, (lds_ptr)&PF[1][0], 4,
+                                               (int)b_off(pc0 + w * 32 + (lane & 31), k), 0, 0, 0);
   };
   auto load = [&](const BigTile &c, int k, int8_t *As, int8_t *Bs) {
-    stage_load(plane_rsrc(c.Ap, opb), plane_rsrc(c.Bp, opb), (int)a_off(c.row0, k),
-               (int)b_off(c.col0, k), m0, As, Bs, wave, lane);
+    stage_load(c.ra, c.rb, (int)a_off(c.row0, k), (int)b_off(c.col0, k), m0, As, Bs, wave, lane);
   };
   load(t, 0, A0, B0);
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -576,7 +628,7 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
     const int next = (round + 1) * G + xcd_tile(blockIdx.x, G);
     const bool has_next = next < map.total;
     BigTile tn = t;
-    if (has_next) tn = big_tile(tab, map, next, nmod, kc);
+    if (has_next) tn = big_tile(tab, map, next, nmod, kc, opb);
     if (has_next) load(tn, 0, A0, B0);
     // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v; the
     // lane's word of block (r, c) sits at Cb + 16 c m0 + 16 r
@@ -864,13 +916,13 @@ hipError_t launch_residue_gemm_bf16(const RGemm *tab, int count, int m, int leve
 }
 
 hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, int nmod,
-                               const OzakiConsts *kc, hipStream_t s) {
+                               const OzakiConsts *kc, hipStream_t s, int tile) {
   if (count <= 0 || m <= 0) return hipSuccess;
   if (nmod < 2 || nmod > OZ_MAX_MODULI) return hipErrorInvalidValue;
   const int m0 = m >> levels;
   if (levels < 0 || (m0 << levels) != m) return hipErrorInvalidValue;
   bool ok;
-  if (m0 % JBM == 0 && (long long)m * m < (1LL << 31)) {
+  if (tile == 0 && m0 % JBM == 0 && (long long)m * m < (1LL << 31)) {
     // large tiles: one block (8 waves, 128 KB of LDS) per CU; buffer offsets
     // are 32-bit, so an operand quadtree stays below 2 GiB
     TileMap map = make_tilemap(m, JBM, (long long)count * nmod, ok);
@@ -900,6 +952,23 @@ hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, i
   else
     hipLaunchKernelGGL((oz_igemm_kernel<1>), dim3(G), dim3(T256), 0, s, tab, m, nmod, kc, map,
                        m0, levels);
+  return hipGetLastError();
+}
+
+int residue_gemm_check(int m, int levels) {
+  const int m0 = m >> levels;
+  if (m0 % IBM == 0) return 0;
+  if (levels > 0) return -1;  // quadtree operands need whole tiles
+  return m % IBK == 0 ? 2 : (m % 16 == 0 ? 3 : 1);
+}
+
+hipError_t launch_residue_gemm_multi(const char *base, const VSeg *segs, int nseg, int total,
+                                     int nmod, const OzakiConsts *kc, hipStream_t s) {
+  if (nseg <= 0 || total <= 0) return hipSuccess;
+  if (nmod < 2 || nmod > OZ_MAX_MODULI) return hipErrorInvalidValue;
+  const int G = persistent_grid(2, device_cu_count(), total);
+  hipLaunchKernelGGL(oz_igemm_multi_kernel, dim3(G), dim3(T256), 0, s, base, segs, nseg, total,
+                     nmod, kc);
   return hipGetLastError();
 }
 

@@ -115,11 +115,11 @@ int tune_save(const char *path) {
   std::lock_guard<std::mutex> g(t.mu);
   FILE *f = fopen(path, "w");
   if (!f) return set_error(CAPS_EINVAL, "caps_tune_save: cannot open file");
-  fprintf(f, "# lda nrec bs mode_mask max_err -> mode moduli cutoff pattern ms err\n");
+  fprintf(f, "# lda nrec bs mode_mask max_err -> mode moduli cutoff pattern ms err tile\n");
   for (const TuneEntry &e : t.entries)
-    fprintf(f, "%d %d %d %d %.17g %d %d %d %s %.9g %.9g\n", e.lda, e.nrec, e.bs, e.mode_mask,
+    fprintf(f, "%d %d %d %d %.17g %d %d %d %s %.9g %.9g %d\n", e.lda, e.nrec, e.bs, e.mode_mask,
             e.max_err, e.c.mode, e.c.moduli, e.c.cutoff, e.c.pattern[0] ? e.c.pattern : "-",
-            e.c.ms, e.c.err);
+            e.c.ms, e.c.err, e.c.tile);
   fclose(f);
   return CAPS_OK;
 }
@@ -134,6 +134,7 @@ bool valid_entry(const TuneEntry &e, const char *pat) {
                                  : (e.c.moduli != 0 && (e.c.moduli < 2 || e.c.moduli > 16)))
     return false;
   if (e.c.cutoff < 0 || e.c.cutoff > e.nrec) return false;
+  if (e.c.tile != 0 && e.c.tile != 1) return false;
   if (strcmp(pat, "-") == 0) return true;
   if ((int)strlen(pat) != e.nrec) return false;
   for (const char *c = pat; *c; c++)
@@ -141,7 +142,8 @@ bool valid_entry(const TuneEntry &e, const char *pat) {
   return true;
 }
 
-// All-or-nothing: a file with any malformed entry loads nothing.
+// All-or-nothing: a file with any malformed entry loads nothing.  The tile
+// column is optional (0 when absent).
 int tune_load(const char *path) {
   FILE *f = fopen(path, "r");
   if (!f) return set_error(CAPS_EINVAL, "caps_tune_load: cannot open file");
@@ -155,10 +157,11 @@ int tune_load(const char *path) {
     if (*p == '#' || *p == '\n' || *p == '\0') continue;
     TuneEntry e{};
     char pat[64] = {0};
-    if (sscanf(p, "%d %d %d %d %lg %d %d %d %63s %lg %lg", &e.lda, &e.nrec, &e.bs,
+    const int got_fields =
+        sscanf(p, "%d %d %d %d %lg %d %d %d %63s %lg %lg %d", &e.lda, &e.nrec, &e.bs,
                &e.mode_mask, &e.max_err, &e.c.mode, &e.c.moduli, &e.c.cutoff, pat, &e.c.ms,
-               &e.c.err) != 11 ||
-        !valid_entry(e, pat)) {
+               &e.c.err, &e.c.tile);
+    if (got_fields < 11 || !valid_entry(e, pat)) {
       fclose(f);
       char msg[96];
       snprintf(msg, sizeof(msg), "caps_tune_load: malformed entry on line %d", lineno);

@@ -28,6 +28,24 @@ namespace caps {
 
 int set_error(int code, const char *msg);
 
+// b/d patterns over L recursed levels: the prefix patterns b^k d^(L-k) first
+// (k = L .. 0; the learned planner's features count b and d levels, so among
+// patterns it cannot tell apart it keeps the first), then, for L <= 4, every
+// other pattern
+static std::vector<std::string> level_patterns(int L) {
+  std::vector<std::string> out;
+  for (int k = L; k >= 0; k--)
+    out.push_back(std::string((size_t)k, 'b') + std::string((size_t)(L - k), 'd'));
+  if (L <= 4)
+    for (int bits = 0; bits < (1 << L); bits++) {
+      std::string p((size_t)L, 'd');
+      for (int l = 0; l < L; l++)
+        if (bits & (1 << (L - 1 - l))) p[(size_t)l] = 'b';
+      if (std::find(out.begin(), out.end(), p) == out.end()) out.push_back(p);
+    }
+  return out;
+}
+
 std::vector<caps_candidate> enumerate_candidates(const caps_desc &d, int mask) {
   std::vector<caps_candidate> out;
   const int m0 = d.lda >> d.nrec;
@@ -41,24 +59,32 @@ std::vector<caps_candidate> enumerate_candidates(const caps_desc &d, int mask) {
   if ((mask & CAPS_MASK_CRT_INT8) && d.nrec <= 7)
     modes.push_back({CAPS_MODE_CRT_INT8, {12, 13, 14, 15, 16}});
   if (mask & CAPS_MASK_OZAKI_INT8) modes.push_back({CAPS_MODE_OZAKI_INT8, {14, 16}});
+  if ((mask & CAPS_MASK_CRT_BF16) && d.nrec <= 7)
+    modes.push_back({CAPS_MODE_CRT_BF16, {14, 16}});
   for (const ModeSet &ms : modes)
     for (int cutoff = 0; cutoff <= d.nrec; cutoff++) {
       if (cutoff > 0 && (ms.mode == CAPS_MODE_OZAKI_INT8 || m0 % 128 != 0 || !plain_leaf)) break;
       const int L = d.nrec - cutoff;
-      for (int k = L; k >= 0; k--) {  // k leading 'b' levels, then 'd'
-        for (int mod : ms.moduli) {
-          caps_candidate c{};
-          c.mode = ms.mode;
-          c.moduli = mod;
-          c.cutoff = cutoff;
-          std::string p(std::string((size_t)k, 'b') + std::string((size_t)(L - k), 'd') +
-                        std::string((size_t)cutoff, 'b'));
-          snprintf(c.pattern, sizeof(c.pattern), "%s", p.c_str());
-          c.ms = -1;
-          c.err = -1;
-          out.push_back(c);
-        }
-      }
+      // leaf-GEMM block tiles: FP64 128 or 64; INT8 residue products 256 or
+      // 128 where the layout leaf allows the 256 tiles; BF16 one tile
+      int tiles = 1;
+      if (ms.mode == CAPS_MODE_FP64) tiles = 2;
+      if ((ms.mode == CAPS_MODE_CRT_INT8 || ms.mode == CAPS_MODE_OZAKI_INT8) && m0 % 256 == 0)
+        tiles = 2;
+      for (const std::string &p : level_patterns(L))
+        for (int mod : ms.moduli)
+          for (int tile = 0; tile < tiles; tile++) {
+            caps_candidate c{};
+            c.mode = ms.mode;
+            c.moduli = mod;
+            c.cutoff = cutoff;
+            c.tile = tile;
+            snprintf(c.pattern, sizeof(c.pattern), "%s",
+                     (p + std::string((size_t)cutoff, 'b')).c_str());
+            c.ms = -1;
+            c.err = -1;
+            out.push_back(c);
+          }
     }
   return out;
 }
@@ -105,7 +131,11 @@ bool mlp_planner_choose(const caps_desc &d, int mask, double max_err, double spr
   ModelSlot &ms = model();
   std::lock_guard<std::mutex> g(ms.mu);
   if (!ms.m) return false;
-  std::vector<caps_candidate> cands = enumerate_candidates(d, mask);
+  // the model has no tile feature: it scores the default tiles, the
+  // autotuner searches the tile dimension
+  std::vector<caps_candidate> cands;
+  for (const caps_candidate &c : enumerate_candidates(d, mask))
+    if (c.tile == 0) cands.push_back(c);
   if (cands.empty()) return false;
   const int B = (int)cands.size();
   std::vector<float> X((size_t)B * MLP_IN), Y((size_t)B * MLP_OUT);

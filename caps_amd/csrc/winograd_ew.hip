@@ -23,24 +23,22 @@ __device__ inline D2 operator-(const D2 &a, const D2 &b) { return {a.x - b.x, a.
 
 constexpr int EW_THREADS = 256;
 
-// grid.y = sub-problem, grid.x strides over the n elements of a quadrant.
-// Inputs and outputs may alias element-for-element (the DFS register reuse):
-// every lane loads all its inputs before it stores.
+// One entry of a step: block bx of gx strides over the n elements of a
+// quadrant.  Inputs and outputs may alias element-for-element (the DFS
+// register reuse): every lane loads all its inputs before it stores.
 template <int OP, bool VEC2>
-__global__ __launch_bounds__(EW_THREADS) void ew_kernel(const EwPtrs *__restrict__ tab,
-                                                        long long n) {
+__device__ __forceinline__ void ew_body(const EwPtrs &P, long long n, int bx, int gx) {
   using Op = Ew<OP>;
-  const EwPtrs &P = tab[blockIdx.y];
   const double *in[Op::NIN];
   double *out[Op::NOUT];
 #pragma unroll
   for (int k = 0; k < Op::NIN; k++) in[k] = P.in[k];
 #pragma unroll
   for (int k = 0; k < Op::NOUT; k++) out[k] = P.out[k];
-  const long long stride = (long long)gridDim.x * EW_THREADS;
+  const long long stride = (long long)gx * EW_THREADS;
   if (VEC2) {
     const long long n2 = n >> 1;
-    for (long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x; i < n2; i += stride) {
+    for (long long i = (long long)bx * EW_THREADS + threadIdx.x; i < n2; i += stride) {
       D2 x[Op::NIN], y[Op::NOUT];
 #pragma unroll
       for (int k = 0; k < Op::NIN; k++) {
@@ -53,7 +51,7 @@ __global__ __launch_bounds__(EW_THREADS) void ew_kernel(const EwPtrs *__restrict
         reinterpret_cast<double2 *>(out[k])[i] = make_double2(y[k].x, y[k].y);
     }
   } else {
-    for (long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x; i < n; i += stride) {
+    for (long long i = (long long)bx * EW_THREADS + threadIdx.x; i < n; i += stride) {
       double x[Op::NIN], y[Op::NOUT];
 #pragma unroll
       for (int k = 0; k < Op::NIN; k++) x[k] = in[k][i];
@@ -64,20 +62,46 @@ __global__ __launch_bounds__(EW_THREADS) void ew_kernel(const EwPtrs *__restrict
   }
 }
 
+// grid.y = sub-problem, grid.x strides over the n elements of a quadrant
+template <int OP, bool VEC2>
+__global__ __launch_bounds__(EW_THREADS) void ew_kernel(const EwPtrs *__restrict__ tab,
+                                                        long long n) {
+  ew_body<OP, VEC2>(tab[blockIdx.y], n, blockIdx.x, gridDim.x);
+}
+
+// the same over segments of different sizes (one launch for a phase of an
+// uneven batch)
+template <int OP, bool VEC2>
+__global__ __launch_bounds__(EW_THREADS) void ew_multi_kernel(const char *__restrict__ base,
+                                                              const VSeg *__restrict__ segs,
+                                                              int nseg) {
+  const VSeg &S = segs[vseg_find(segs, nseg, blockIdx.x)];
+  const int l = blockIdx.x - S.blk0;
+  const EwPtrs *tab = reinterpret_cast<const EwPtrs *>(base + S.arg);
+  ew_body<OP, VEC2>(tab[l / S.gx], S.n, l % S.gx, S.gx);
+}
+
 template <int OP>
 hipError_t launch_op(const EwPtrs *tab, int count, long long n, bool vec2, hipStream_t s) {
-  long long units = vec2 ? (n >> 1) : n;
-  long long blocks = (units + EW_THREADS - 1) / EW_THREADS;
-  // enough blocks in flight to cover HBM latency: ~8 per CU over the batch
-  long long cap = (long long)device_cu_count() * 8 / (count > 0 ? count : 1);
-  if (cap < 1) cap = 1;
-  if (blocks > cap) blocks = cap;
-  if (blocks < 1) blocks = 1;
-  dim3 grid((unsigned)blocks, (unsigned)count);
+  int gx, gy;
+  ew_grid(count, n, vec2, gx, gy);
+  dim3 grid((unsigned)gx, (unsigned)gy);
   if (vec2)
     hipLaunchKernelGGL((ew_kernel<OP, true>), grid, dim3(EW_THREADS), 0, s, tab, n);
   else
     hipLaunchKernelGGL((ew_kernel<OP, false>), grid, dim3(EW_THREADS), 0, s, tab, n);
+  return hipGetLastError();
+}
+
+template <int OP>
+hipError_t launch_op_multi(const char *base, const VSeg *segs, int nseg, int blocks, bool vec2,
+                           hipStream_t s) {
+  if (vec2)
+    hipLaunchKernelGGL((ew_multi_kernel<OP, true>), dim3(blocks), dim3(EW_THREADS), 0, s, base,
+                       segs, nseg);
+  else
+    hipLaunchKernelGGL((ew_multi_kernel<OP, false>), dim3(blocks), dim3(EW_THREADS), 0, s, base,
+                       segs, nseg);
   return hipGetLastError();
 }
 
@@ -95,6 +119,34 @@ int ew_additions(int op) {
     case EW_DFS_POST3: return 1;
   }
   return 0;
+}
+
+void ew_grid(int count, long long n, bool vec2, int &gx, int &gy) {
+  long long units = vec2 ? (n >> 1) : n;
+  long long blocks = (units + EW_THREADS - 1) / EW_THREADS;
+  // enough blocks in flight to cover HBM latency: ~8 per CU over the batch
+  long long cap = (long long)device_cu_count() * 8 / (count > 0 ? count : 1);
+  if (cap < 1) cap = 1;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  gx = (int)blocks;
+  gy = count;
+}
+
+hipError_t launch_ew_multi(int op, const char *base, const VSeg *segs, int nseg, int blocks,
+                           bool vec2, hipStream_t s) {
+  if (nseg <= 0 || blocks <= 0) return hipSuccess;
+  switch (op) {
+    case EW_BFS_PRE: return launch_op_multi<EW_BFS_PRE>(base, segs, nseg, blocks, vec2, s);
+    case EW_BFS_POST: return launch_op_multi<EW_BFS_POST>(base, segs, nseg, blocks, vec2, s);
+    case EW_DFS_PRE1: return launch_op_multi<EW_DFS_PRE1>(base, segs, nseg, blocks, vec2, s);
+    case EW_DFS_PRE2: return launch_op_multi<EW_DFS_PRE2>(base, segs, nseg, blocks, vec2, s);
+    case EW_DFS_PRE3: return launch_op_multi<EW_DFS_PRE3>(base, segs, nseg, blocks, vec2, s);
+    case EW_DFS_POST1: return launch_op_multi<EW_DFS_POST1>(base, segs, nseg, blocks, vec2, s);
+    case EW_DFS_POST2: return launch_op_multi<EW_DFS_POST2>(base, segs, nseg, blocks, vec2, s);
+    case EW_DFS_POST3: return launch_op_multi<EW_DFS_POST3>(base, segs, nseg, blocks, vec2, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_ew(int op, const EwPtrs *tab, int count, long long n, bool vec2,

@@ -98,7 +98,10 @@ def main(argv=None) -> int:
         C = torch.empty_like(A)
         Ref = torch.empty_like(A)
         caps_amd.reference_product(A, B, Ref, desc)
-        cands = caps_amd.candidates(desc, mask)
+        # the model's features count 'b' and 'd' levels and carry no tile:
+        # the data keeps the prefix patterns b^k d^l and the default tiles
+        cands = [c for c in caps_amd.candidates(desc, mask)
+                 if c.tile == 0 and "db" not in c.pattern.decode()[:r - c.cutoff]]
         if a.crt_moduli:
             keep = {int(x) for x in a.crt_moduli.split(",")}
             cands = [c for c in cands if c.mode != caps_amd.MODE_CRT_INT8 or c.moduli in keep]

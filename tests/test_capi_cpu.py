@@ -85,15 +85,27 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
 
 def test_candidate_enumeration(lib):
     D = caps_amd.MatDescriptor.one_device
-    # n = 8192, nrec = 2 (leaves of 2048): cutoffs 0..2, prefix patterns b^k d^(L-k)
+    # n = 8192, nrec = 2 (leaves of 2048): cutoffs 0..2, every b/d pattern of
+    # the recursed levels, both FP64 block tiles
     fp = caps_amd.candidates(D(8192, 2, 1), caps_amd.MASK_FP64)
-    assert sorted((c.cutoff, c.pattern.decode()) for c in fp) == sorted(
-        [(0, "bb"), (0, "bd"), (0, "dd"), (1, "bb"), (1, "db"), (2, "bb")])
+    assert sorted((c.cutoff, c.pattern.decode(), c.tile) for c in fp) == sorted(
+        (cut, p, t) for cut, p in [(0, "bb"), (0, "bd"), (0, "db"), (0, "dd"), (1, "bb"),
+                                   (1, "db"), (2, "bb")] for t in (0, 1))
+    # prefix patterns come first (the learned planner keeps the first of equal features)
+    assert [c.pattern.decode() for c in fp if c.cutoff == 0 and c.tile == 0] == [
+        "bb", "bd", "dd", "db"]
     crt = caps_amd.candidates(D(8192, 2, 1), caps_amd.MASK_CRT_INT8)
-    assert {c.moduli for c in crt} == {12, 13, 14, 15, 16} and len(crt) == 30
+    # 256-aligned leaves: INT8 tiles 256 and 128
+    assert {c.moduli for c in crt} == {12, 13, 14, 15, 16} and len(crt) == 7 * 5 * 2
+    assert {c.tile for c in crt} == {0, 1}
     # leaves not a multiple of 128: no cutoff
     small = caps_amd.candidates(D(392, 3, 7), caps_amd.MASK_FP64 | caps_amd.MASK_CRT_INT8)
     assert all(c.cutoff == 0 for c in small)
+    # 8 patterns; FP64 two tiles, residue one (leaves of 49 take the 128 tiles)
+    assert len(small) == 8 * 2 + 8 * 5
+    deep = caps_amd.candidates(D(8192, 6, 1), caps_amd.MASK_FP64)
+    # beyond 4 recursed levels only the prefix patterns
+    assert len({c.pattern.decode() for c in deep if c.cutoff == 0}) == 7
     f = caps_amd.candidate_features(D(8192, 2, 1), crt[0], 10.0)
     # mode ordinal 1 (residue INT8); leaves of 2048: alignment log2(256) / 8 = 1
     assert list(f[:1]) == [13.0] and f[1] == 10.0 and f[6] == 1.0 and f[7] == 1.0

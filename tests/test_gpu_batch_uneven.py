@@ -83,7 +83,7 @@ def test_uneven_batch_fp64_bit_identical_to_single_calls():
 
 def test_uneven_batch_residue_mode_error_bar():
     sizes = batch_plan.uneven_sizes()
-    plans = batch_plan.choose_plans(sizes, caps_amd.MODE_CRT_INT8, max_err=1e-14)
+    plans = batch_plan.choose_plans(sizes, caps_amd.MODE_CRT_INT8, max_err=1e-14, moduli=14)
     K = next(iter(plans.values())).moduli
     print(f"[gpu] residue batch: {K} moduli, depths "
           f"{sorted(set(p.nrec for p in plans.values()))}", flush=True)
