@@ -4,6 +4,7 @@ device-resident inputs, then a plain device-to-device copy of known size
 WRITE_SIZE: the copy moves exactly --copy-mib MiB in and out).
 
     python -m caps_amd.profile_run --n 8192 --nrec 2 --pattern bb --mode crt --calls 3
+    python -m caps_amd.profile_run --n 8192 --nrec 2 --bs 8 --mode layout --calls 20
     rocprofv3 --kernel-trace --stats --output-format csv -d OUT -o run -- \\
         python3 -m caps_amd.profile_run --mode crt --calls 3
 
@@ -17,7 +18,7 @@ import sys
 import caps_amd
 
 MODES = {"fp64": caps_amd.MODE_FP64, "crt": caps_amd.MODE_CRT_INT8,
-         "leaf": caps_amd.MODE_OZAKI_INT8}
+         "leaf": caps_amd.MODE_OZAKI_INT8, "layout": None}
 
 
 def main(argv=None) -> int:
@@ -54,8 +55,13 @@ def main(argv=None) -> int:
     moduli = 0 if mode == caps_amd.MODE_FP64 else a.moduli
     flags = caps_amd.FLAG_NO_GRAPH if a.no_graph else 0
     for i in range(a.calls):
-        caps_amd.multiply_device(dA, dB, dC, desc, a.pattern, mode=mode, moduli=moduli,
-                                 flags=flags)
+        if mode is None:
+            # column-major -> CAPS layout and back (16 bytes per element each way)
+            caps_amd.to_layout_device(dA, dC, desc)
+            caps_amd.from_layout_device(dC, dB, desc)
+        else:
+            caps_amd.multiply_device(dA, dB, dC, desc, a.pattern, mode=mode, moduli=moduli,
+                                     flags=flags)
         torch.cuda.synchronize()
         print(f"[profile_run] {a.mode} n={n} nrec={a.nrec} pattern={a.pattern} call {i + 1}",
               file=sys.stderr, flush=True)

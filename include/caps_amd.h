@@ -104,6 +104,10 @@ typedef struct caps_opts {
 /* Run two leading 'b' levels as separate passes instead of the fused
  * two-level kernels (same results; for testing and timing comparisons). */
 #define CAPS_FLAG_NO_FUSION 8
+/* Leaf GEMMs on the smaller block tile: INT8 residue products 128 x 128
+ * instead of 256 x 256, FP64 64 x 64 instead of 128 x 128 (same results; the
+ * autotuner's tile dimension, caps_candidate.tile). */
+#define CAPS_FLAG_SMALL_TILE 16
 
 /* Select the device (HIP ordinal) for the calling thread and check that it
  * is gfx950.  Optional: the first call of any entry point does it for the
@@ -209,7 +213,8 @@ typedef struct caps_candidate {
   int mode;          /* caps_mode                                           */
   int moduli;        /* split modes                                         */
   int cutoff;        /* caps_opts.cutoff                                    */
-  int reserved;
+  int tile;          /* 0: default leaf-GEMM block tile, 1: the smaller one
+                        (CAPS_FLAG_SMALL_TILE)                              */
   char pattern[32];  /* one char per level, NUL-terminated                  */
   double ms;         /* median milliseconds per multiply                    */
   double err;        /* normwise error vs the reference product (-1: none)  */
@@ -231,9 +236,10 @@ int caps_normwise_error(const double *dC, const double *dRef, const caps_desc *d
  * `stream`) and, if dRef is not NULL, its error; fills cand->ms and ->err. */
 int caps_measure(const double *dA, const double *dB, double *dC, const caps_desc *desc,
                  const double *dRef, int reps, caps_candidate *cand, void *stream);
-/* Exhaustive search over the candidates allowed by mode_mask (pattern over
- * the recursed levels, cutoff, moduli 12..16 for split modes) on the given
- * operands; all measured candidates go to out[0..*n_out) (up to max_out),
+/* Exhaustive search over the candidates allowed by mode_mask (every b/d
+ * pattern over the recursed levels up to 4 levels, prefix patterns b^k d^l
+ * beyond, cutoff, moduli 12..16 for split modes, and the leaf-GEMM block tile)
+ * on the given operands; all measured candidates go to out[0..*n_out) (up to max_out),
  * the fastest one with err <= max_err to *best and to the tuning table. */
 int caps_autotune(const double *dA, const double *dB, double *dC, const caps_desc *desc,
                   int mode_mask, double max_err, int reps, caps_candidate *out, int max_out,

@@ -51,7 +51,11 @@ def _setup():
 
 # ------------------------------------------------------------------ layout
 @pytest.mark.parametrize("n,r,bs", [(28, 2, 7), (392, 3, 7), (64, 2, 16), (1024, 3, 1),
-                                    (48, 0, 12), (56, 3, 1)])
+                                    (48, 0, 12), (56, 3, 1),
+                                    # LDS-staged 64 x 64 tiles: bs dividing 64, bs a
+                                    # multiple of 64, bs = leaf
+                                    (512, 2, 8), (512, 1, 32), (256, 1, 2), (1024, 1, 64),
+                                    (512, 0, 128), (1024, 2, 256), (768, 1, 4), (192, 0, 1)])
 def test_layout_device_matches_oracle(n, r, bs):
     rng = np.random.default_rng(n + r)
     M = rng.standard_normal((n, n))
@@ -128,6 +132,22 @@ def test_unit_leaves_bit_exact_on_reals(n, r, pattern):
 
 
 # ------------------------------------------------------------------ configurations
+@pytest.mark.parametrize("n,r,bs", [(512, 2, 8), (1024, 2, 64), (768, 1, 4), (448, 3, 7)])
+def test_blocked_leaves_give_the_column_major_bits(n, r, bs):
+    """1 < bs < leaf: block_multiply reorders the leaves to column-major around
+    the dgemm (multiply.cpp:1093-1139); a pure permutation, so the product of
+    the same matrices stored with bs blocks has the bits of the bs = 1 run."""
+    rng = np.random.default_rng(n + bs)
+    A = rng.uniform(-1, 1, (n, n))
+    B = rng.uniform(-1, 1, (n, n))
+    pat = "b" + "d" * (r - 1)
+    c1 = oracle.from_caps(run(oracle.to_caps(A, r, 1), oracle.to_caps(B, r, 1), n, r, 1, pat),
+                          n, r, 1)
+    cb = oracle.from_caps(run(oracle.to_caps(A, r, bs), oracle.to_caps(B, r, bs), n, r, bs, pat),
+                          n, r, bs)
+    assert np.array_equal(c1, cb)
+
+
 def test_cfg1_ddd_integers_exact_and_reals_close():
     """n=1024, r=3, 'ddd', bs=1: [-8,8] integers exact; uniform [-1,1) from
     drand48 (randombenchmark.cpp:19-22) within 1e-14 normwise of Dot2."""
