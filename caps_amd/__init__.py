@@ -112,6 +112,8 @@ SIGNATURES = {
     "caps_num_entries": (ctypes.c_longlong, [_D]),
     "caps_multiply": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_char_p, _O]),
     "caps_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_char_p, _O, _vp]),
+    "caps_multiply_device_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _D, ctypes.c_char_p, _O,
+                                                    _vp]),
     "caps_multiply_batch_device": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                                   _vp]),
     "caps_to_layout": (ctypes.c_int, [_vp, _vp, _D]),
@@ -243,6 +245,22 @@ def multiply_device(A, B, C, desc: MatDescriptor, pattern: str | None = None,
     _check(load().caps_multiply_device(_dev_ptr(A, n2), _dev_ptr(B, n2), _dev_ptr(C, n2),
                                        ctypes.byref(desc), _pattern(pattern), ctypes.byref(o),
                                        _stream_ptr(stream)))
+
+
+NOALLOC_SCRATCH_MIB = 256  # CAPS_NOALLOC_SCRATCH_MIB
+
+
+def multiply_device_noalloc(A, B, C, desc: MatDescriptor, pattern: str | None = None,
+                            mode: int = MODE_FP64, moduli: int = 0, flags: int = 0,
+                            stream=None, cutoff: int = 0) -> None:
+    """multiply_device without device allocation: tables and workspace in the
+    library's fixed per-device scratch (NOALLOC_SCRATCH_MIB MiB); calls that
+    overlap on one device race."""
+    n2 = desc.lda * desc.lda
+    o = _opts(mode, moduli, flags, cutoff)
+    _check(load().caps_multiply_device_noalloc(_dev_ptr(A, n2), _dev_ptr(B, n2), _dev_ptr(C, n2),
+                                               ctypes.byref(desc), _pattern(pattern),
+                                               ctypes.byref(o), _stream_ptr(stream)))
 
 
 def multiply_batch_device(As, Bs, Cs, descs, patterns=None, mode: int = MODE_FP64,

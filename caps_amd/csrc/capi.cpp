@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -21,6 +22,12 @@
 #include "ozaki.h"
 #include "planner.h"
 #include "tune.h"
+
+// Device scratch of caps_multiply_device_noalloc: every GPU the library runs
+// on holds its own copy from module load until the process ends.  Host code
+// never uses this name as an address: the per-device address comes from
+// hipGetSymbolAddress (State::scratch).
+static __device__ __align__(256) char scratch[(size_t)CAPS_NOALLOC_SCRATCH_MIB << 20];
 
 namespace caps {
 
@@ -68,6 +75,10 @@ struct State {
   double *stage[3] = {nullptr, nullptr, nullptr};
   size_t stage_elems = 0;
   hipStream_t stage_stream = nullptr;
+  // caps_multiply_device_noalloc: the address of each device's scratch array,
+  // and host-only plans (their tables are copied into the scratch per call)
+  std::map<int, char *> scratch;
+  std::vector<std::unique_ptr<Plan>> host_plans;
 };
 
 State &st() {
@@ -89,6 +100,14 @@ int ensure_device_locked(State &s) {
   if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(CAPS_ENODEV, "device %d is %s, this library is built for gfx950", dev,
                 prop.gcnArchName);
+  if (!s.scratch.count(dev)) {  // once per device, with that device current
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CAPS_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    s.scratch[dev] = static_cast<char *>(p);
+  }
   s.device = dev;
   s.cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   return CAPS_OK;
@@ -413,6 +432,91 @@ int caps_multiply_device(const double *dA, const double *dB, double *dC, const c
   int rc = ensure_device_locked(s);
   if (rc) return rc;
   return multiply_device_locked(s, dA, dB, dC, desc, pattern, opts, (hipStream_t)stream);
+}
+
+int caps_multiply_device_noalloc(const double *dA, const double *dB, double *dC,
+                                 const caps_desc *desc, const char *pattern,
+                                 const caps_opts *opts, void *stream) {
+  State &s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  int rc = ensure_device_locked(s);
+  if (rc) return rc;
+  rc = check_desc(desc);
+  if (rc) return rc;
+  if (!dA || !dB || !dC) return fail(CAPS_EINVAL, "NULL matrix pointer");
+  if (opts && opts->mode == CAPS_MODE_AUTO)
+    return fail(CAPS_EUNSUPPORTED, "caps_multiply_device_noalloc takes an explicit leaf mode");
+  int mode, moduli;
+  rc = resolve_mode(*desc, opts, mode, moduli);
+  if (rc) return rc;
+  const int cutoff = opts ? opts->cutoff : 0;
+  const size_t cap = (size_t)CAPS_NOALLOC_SCRATCH_MIB << 20;
+  // an automatic pattern must fit the scratch as well as the budget
+  const size_t budget =
+      std::min(planner_budget_bytes(s.available_doubles, s.budget_set), cap);
+  std::string pat;
+  rc = resolve_pattern(*desc, pattern, opts, pat, mode, moduli,
+                       (long long)(budget / sizeof(double)), true);
+  if (rc) return rc;
+  rc = check_cutoff(*desc, mode, cutoff);
+  if (rc) return rc;
+  const bool fuse = !(opts && (opts->flags & CAPS_FLAG_NO_FUSION));
+  const int tile = (opts && (opts->flags & CAPS_FLAG_SMALL_TILE)) ? 1 : 0;
+  hipStream_t hs = (hipStream_t)stream;
+  char *base = s.scratch[s.device];
+  // host-only plans, one per (buffers, shape, options, device): their tables
+  // hold this device's scratch addresses
+  PlanKey key{dA, dB, dC, *desc, pat, mode, moduli, cutoff, fuse,
+              "scratch of device " + std::to_string(s.device), nullptr, tile};
+  Plan *p = nullptr;
+  for (auto &hp : s.host_plans)
+    if (hp->key == key) {
+      p = hp.get();
+      p->last_use = ++s.clock;
+    }
+  if (!p) {
+    auto plan = std::make_unique<Plan>();
+    plan->key = key;
+    if (build_plan(*plan, dA, dB, dC, *desc, pat, mode, moduli, cutoff, nullptr, fuse, tile) !=
+        CAPS_OK)
+      return fail(CAPS_EINVAL, "internal: plan layout check failed");
+    const size_t toff = (plan->host_tables.size() + 255) / 256 * 256;
+    if (toff + plan->work_bytes > cap)
+      return fail(CAPS_ENOMEM,
+                  "plan needs %.1f MiB of tables and workspace; caps_multiply_device_noalloc "
+                  "has %d MiB of scratch",
+                  (toff + plan->work_bytes) / 1048576.0, CAPS_NOALLOC_SCRATCH_MIB);
+    if (build_plan(*plan, dA, dB, dC, *desc, pat, mode, moduli, cutoff, base + toff, fuse,
+                   tile) != CAPS_OK ||
+        (plan->host_tables.size() + 255) / 256 * 256 != toff)
+      return fail(CAPS_EINVAL, "internal: plan layout check failed");
+    plan->d_tables = base;
+    plan->d_work = base + toff;
+    plan->last_use = ++s.clock;
+    // host memory only; keep the 16 most recent
+    if (s.host_plans.size() >= 16) {
+      auto oldest = std::min_element(
+          s.host_plans.begin(), s.host_plans.end(),
+          [](const std::unique_ptr<Plan> &a, const std::unique_ptr<Plan> &b) {
+            return a->last_use < b->last_use;
+          });
+      s.host_plans.erase(oldest);
+    }
+    p = plan.get();
+    s.host_plans.push_back(std::move(plan));
+  }
+  // the tables go to the front of the scratch in stream order (a pageable
+  // source is read before the call returns), then the steps run eagerly
+  HIP_TRY(hipMemcpyAsync(base, p->host_tables.data(), p->host_tables.size(),
+                         hipMemcpyHostToDevice, hs));
+  hipError_t e = run_steps(*p, hs);
+  if (e != hipSuccess) return fail(CAPS_EHIP, "kernel launch: %s", hipGetErrorString(e));
+  s.additions += p->additions;
+  s.multiplications += p->multiplications;
+  s.leaves += p->leaves;
+  s.peak_mb = p->work_bytes / 1048576.0;
+  s.exec_pattern = p->exec_pattern;
+  return CAPS_OK;
 }
 
 int caps_multiply(const double *A, const double *B, double *C, const caps_desc *desc,
@@ -781,6 +885,7 @@ int caps_release(void) {
   State &s = st();
   std::lock_guard<std::mutex> g(s.mu);
   evict_locked(s, 0);
+  s.host_plans.clear();
   for (auto &p : s.stage)
     if (p) {
       hipFree(p);

@@ -139,6 +139,22 @@ int caps_multiply_device(const double *dA, const double *dB, double *dC,
                          const caps_desc *desc, const char *pattern,
                          const caps_opts *opts, void *stream);
 
+/* Same as caps_multiply_device, for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops): it calls no
+ * hipMalloc / hipFree.  No counterpart in the reference.  The pointer tables
+ * and workspace live in a scratch array of CAPS_NOALLOC_SCRATCH_MIB MiB that
+ * every device the library runs on holds from module load until the process
+ * ends; a call whose plan does not fit returns CAPS_ENOMEM (e.g. FP64 n = 2048
+ * 'bd' fits, residue mode n = 1024 'bb' fits).  No graph: the steps are
+ * enqueued one by one on `stream`.  CAPS_MODE_AUTO is not accepted.  All calls
+ * on one device share its scratch (it keeps the last call's contents; no step
+ * reads scratch it has not written in the same call): calls that overlap on
+ * one device, on different streams, race. */
+#define CAPS_NOALLOC_SCRATCH_MIB 256
+int caps_multiply_device_noalloc(const double *dA, const double *dB, double *dC,
+                                 const caps_desc *desc, const char *pattern,
+                                 const caps_opts *opts, void *stream);
+
 /* `count` independent multiplies C_i = A_i * B_i of possibly different
  * sizes and depths (device pointers, CAPS layout each), patterns[i] or NULL
  * per item.  opts is NULL (defaults) or an array of `count` caps_opts: one
