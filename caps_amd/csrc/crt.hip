@@ -125,10 +125,17 @@ __global__ __launch_bounds__(T) void crt_finalize_kernel(const CrtMats *__restri
 // ------------------------------------------------------------------ split
 // A leaf L -> row-major residue leaves; 64 x 64 tiles transposed through LDS.
 // FAST: m % 64 == 0 and the leaf is column-major (bs == 1 or bs == m).
+// LDS: column c of the tile in Ts[c], row r at slot r ^ tswz(c).  The
+// transposed read has lanes on 16 rows and 4 column groups (c >> 4); the
+// swizzle puts the four groups 8 slots apart, so any 16 or 32 consecutive
+// lanes read distinct 8-byte slots of a 512-byte span, and the column-wise
+// write is a permutation of one 512-byte column.
+__device__ __forceinline__ int tswz(int c) { return ((c >> 4) & 3) << 3; }
+
 template <bool FAST>
 __device__ __forceinline__ void split_a_body(const CrtMats *__restrict__ M,
                                              const OzakiConsts *__restrict__ kc, int bx, int by,
-                                             double (&Ts)[64][65]) {
+                                             double (&Ts)[64][64]) {
   const int m = M->m, nmod = M->nmod, bs = M->bs;
   const int L = by;
   int bi, bj;
@@ -144,7 +151,7 @@ __device__ __forceinline__ void split_a_body(const CrtMats *__restrict__ M,
       v = M->A[lbase + li + (long long)lj * m];
     else if (li < m && lj < m)
       v = M->A[lbase + leaf_elem_offset(m, bs, li, lj)];
-    Ts[c][r] = v;
+    Ts[c][r ^ tswz(c)] = v;
   }
   __syncthreads();
   const int r = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
@@ -153,7 +160,7 @@ __device__ __forceinline__ void split_a_body(const CrtMats *__restrict__ M,
   const int sa = M->sa[bi * m + li];
   double x[16];
 #pragma unroll
-  for (int c = 0; c < 16; c++) x[c] = trunc(ldexp(Ts[kq + c][r], sa));
+  for (int c = 0; c < 16; c++) x[c] = trunc(ldexp(Ts[kq + c][r ^ tswz(kq)], sa));
   const long long plane = (long long)M->n * M->n;
   int8_t *dst = M->ra + lbase + (long long)li * m + k0 + kq;
   for (int l = 0; l < nmod; l++) {
@@ -182,7 +189,7 @@ __device__ __forceinline__ void split_a_body(const CrtMats *__restrict__ M,
 template <bool FAST>
 __global__ __launch_bounds__(T) void crt_split_a_kernel(const CrtMats *__restrict__ M,
                                                         const OzakiConsts *__restrict__ kc) {
-  __shared__ double Ts[64][65];  // [col][row]
+  __shared__ double Ts[64][64];  // [col][swizzled row]
   split_a_body<FAST>(M, kc, blockIdx.x, blockIdx.y, Ts);
 }
 
@@ -305,7 +312,7 @@ template <int K, bool F, int NP>
 __global__ __launch_bounds__(T) void crt_multi_kernel(const char *__restrict__ base,
                                                       const VSeg *__restrict__ segs, int nseg,
                                                       const OzakiConsts *__restrict__ kc) {
-  __shared__ double Ts[K == CRT_SPLIT_A ? 64 : 1][K == CRT_SPLIT_A ? 65 : 1];
+  __shared__ double Ts[K == CRT_SPLIT_A ? 64 : 1][K == CRT_SPLIT_A ? 64 : 1];
   const VSeg &S = segs[vseg_find(segs, nseg, blockIdx.x)];
   const int l = blockIdx.x - S.blk0;
   const int bx = l % S.gx, by = l / S.gx;

@@ -12,7 +12,9 @@
 //
 // HBM-bound: per position, FP64 pre reads 16 x 8 B and writes 40 x 8 B per
 // side; FP64 post reads 49 x 8 B and writes 16 x 8 B.  The residue post is
-// fused with the CRT reconstruction: 49 x nmod bytes in, 16 FP64 results out.
+// fused with the CRT reconstruction: 49 x nmod bytes in, 16 FP64 results out
+// (its VALU work, about 5,000 lane instructions per position for 14 moduli,
+// is close to its memory time).
 #include "ew_ops.h"
 #include "fused2.h"
 #include "ozaki_dev.h"
@@ -24,8 +26,11 @@ namespace {
 
 constexpr int T = 256;
 
-// global-memory views (stores through them are global_*, not flat_*)
+// global-memory views (accesses through them are global_*, not flat_*)
 typedef __attribute__((address_space(1))) unsigned short gushort;
+typedef const __attribute__((address_space(1))) unsigned gcuint;
+typedef __attribute__((address_space(1))) double gdouble;
+typedef const __attribute__((address_space(1))) int gcint;
 
 // level-1 operands at one sub-position from the four quadrant values (order
 // 11, 21, 12, 22) -> T1..T7 (A side) or S1..S7 (B side)
@@ -255,7 +260,8 @@ __global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restric
 
 // ---------------------------------------------------------------- residue post + CRT
 // One block per PPOS consecutive positions of the depth-2 sub-blocks, two
-// phases through LDS:
+// phases through LDS (each product plane is read in PPOS-byte segments:
+// 128 keeps them whole cache lines):
 //   1. lane per (4 positions, modulus): 49 product words, both levels of the
 //      post-combination, the 16 residues of C's sub-blocks into LDS;
 //   2. lane per (position, sub-block of C), consecutive lanes on consecutive
@@ -275,10 +281,13 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
     const long long u4 = pos0 / 4 + g;
     if (4 * u4 >= q) continue;  // q % 4 == 0
     const int mi = kc->m[l], c22 = kc->c22[l];
-    const char *qb = reinterpret_cast<const char *>(P->q) + l * P->sq;
+    // this lane's word of product 0; product k lies k * step bytes further
+    // (a wave-uniform offset)
+    const char *qb = reinterpret_cast<const char *>(P->q) + l * P->sq + 4 * u4;
+    const long long step = P->step;
     unsigned w[49];
 #pragma unroll
-    for (int k = 0; k < 49; k++) w[k] = reinterpret_cast<const unsigned *>(qb + k * P->step)[u4];
+    for (int k = 0; k < 49; k++) w[k] = *(gcuint *)(qb + k * step);
     unsigned out[16];
 #pragma unroll
     for (int o = 0; o < 16; o++) out[o] = 0;
@@ -334,7 +343,8 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
     const int gi = (aa & 1) * 2 * sub + (bb & 1) * sub + ri;
     const int gj = (aa >> 1) * 2 * sub + (bb >> 1) * sub + cj;
     const double X = crt_finish<NP>(S, kc);
-    P->out_c[o * q + dst_in_sub] = ldexp(X, -(P->sa[gi] + P->sb[gj]));
+    ((gdouble *)P->out_c)[o * q + dst_in_sub] =
+        ldexp(X, -(((gcint *)P->sa)[gi] + ((gcint *)P->sb)[gj]));
   }
 }
 

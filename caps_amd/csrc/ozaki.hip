@@ -197,6 +197,7 @@ constexpr int ISTR = IBK + 16;  // LDS row stride (bytes): b128 reads of 16 lane
 // (m % 16 == 0, so a chunk is wholly inside or outside).  Checks use the
 // slice origin; plain operands only.
 typedef __attribute__((address_space(1))) unsigned gu32;  // stores as global_*, not flat_*
+typedef __attribute__((address_space(1))) v4i gv4i;
 
 template <int CHECK>
 __device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ab,
@@ -509,6 +510,10 @@ __device__ __forceinline__ BigTile big_tile(const RGemm *__restrict__ tab, const
   return t;
 }
 
+// SMALL_K: K = m <= 16384, so |sums| <= m 2^14 < 2^28 and the one-pass
+// exact reduction res_byte128 applies (sums start at 128); larger K takes the
+// general one
+template <bool SMALL_K>
 __global__ __launch_bounds__(JT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
                     const OzakiConsts *__restrict__ kc, TileMap map, int m0, int d) {
@@ -522,10 +527,7 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
   const int l16 = lane & 15, lk = lane >> 4;
   const int G = gridDim.x;
   const int nk = m / JBK;  // even: m0 % 256 == 0
-  // |sums| <= m 2^14 < 2^28 for m <= 16384: the one-pass exact reduction
-  // res_byte128 (sums start at 128); larger K takes the general one
-  const bool small_k = m <= 16384;
-  const int bias = small_k ? 128 : 0;
+  constexpr int bias = SMALL_K ? 128 : 0;
   // read offsets (bytes) of this lane's row l16, chunk lk (kh = 0) and lk + 4 (kh = 1)
   const int s16 = jswz(l16);
   const int rd0 = l16 * JBK + ((lk ^ s16) << 4);
@@ -630,12 +632,18 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
     BigTile tn = t;
     if (has_next) tn = big_tile(tab, map, next, nmod, kc, opb);
     if (has_next) load(tn, 0, A0, B0);
-    // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v; the
-    // lane's word of block (r, c) sits at Cb + 16 c m0 + 16 r
-    int8_t *Cb = t.Cp + quad_leaf(t.row0 / m0, t.col0 / m0, d) * mm0 +
-                 (long long)((t.col0 % m0) + wn * 64 + l16) * m0 + (t.row0 % m0) + wm * 128 +
-                 4 * lk;
-    if (small_k) {
+    // D layout: lane holds column (lane % 16), rows 4 * (lane / 16) + v of
+    // each 16 x 16 block.  The tile's C bytes go through LDS (A1: columns
+    // 0..127, B1: 128..255; column c at word 64 c, its word w at w ^ 4 (c &
+    // 15), so the 16 columns of one store hit distinct banks) and leave as
+    // whole 256-byte columns, 16 bytes per lane.
+    unsigned *Ch = reinterpret_cast<unsigned *>(wn < 2 ? A1 : B1);
+    auto put = [&](int r, int c, unsigned v) {
+      const int col = (wn & 1) * 64 + c * 16 + l16;
+      const int w = wm * 32 + r * 4 + lk;
+      Ch[col * 64 + (w ^ (4 * l16))] = v;
+    };
+    if constexpr (SMALL_K) {
       const int c14m = t.c14 - 16384;
       const float cl = 0.5f - 128.0f * t.invf;
 #pragma unroll
@@ -645,7 +653,7 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
           unsigned v = 0;
 #pragma unroll
           for (int q = 0; q < 4; q++) v = res_byte128(acc[r][c][q], c14m, t.mf, t.invf, cl, q, v);
-          *(gu32 *)(Cb + (long long)(16 * c) * m0 + 16 * r) = v ^ 0x80808080u;
+          put(r, c, v ^ 0x80808080u);
         }
     } else {
 #pragma unroll
@@ -656,8 +664,23 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
 #pragma unroll
           for (int q = 0; q < 4; q++)
             v |= (unsigned)(mod_i32(acc[r][c][q], t.mi, t.invf) & 0xff) << (8 * q);
-          *(gu32 *)(Cb + (long long)(16 * c) * m0 + 16 * r) = v;
+          put(r, c, v);
         }
+    }
+    // LDS writes done before the barrier; the next tile's first stage (in
+    // A0 / B0) may still be loading
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    stage_barrier();
+    // lane: 16 bytes (chunk j16) of tile column pass * 32 + (tid >> 4)
+    const int j16 = tid & 15;
+    int8_t *Cl = t.Cp + quad_leaf(t.row0 / m0, t.col0 / m0, d) * mm0 +
+                 (long long)(t.col0 % m0 + (tid >> 4)) * m0 + (t.row0 % m0) + 16 * j16;
+#pragma unroll
+    for (int pass = 0; pass < 8; pass++) {
+      const int ccl = (pass * 32 + (tid >> 4)) & 127;  // pass < 4: A1, else B1
+      const unsigned *Cs = reinterpret_cast<const unsigned *>(pass < 4 ? A1 : B1);
+      const v4i v = *reinterpret_cast<const v4i *>(Cs + ccl * 64 + ((4 * j16) ^ (4 * (ccl & 15))));
+      *(gv4i *)(Cl + pass * 32 * m0) = v;
     }
     if (!has_next) break;
     t = tn;
@@ -928,8 +951,12 @@ hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, i
     TileMap map = make_tilemap(m, JBM, (long long)count * nmod, ok);
     if (!ok) return hipErrorInvalidValue;
     const int G = persistent_grid(1, device_cu_count(), map.total);
-    hipLaunchKernelGGL(oz_igemm_big_kernel, dim3(G), dim3(JT), 0, s, tab, m, nmod, kc, map, m0,
-                       levels);
+    if (m <= 16384)
+      hipLaunchKernelGGL((oz_igemm_big_kernel<true>), dim3(G), dim3(JT), 0, s, tab, m, nmod, kc,
+                         map, m0, levels);
+    else
+      hipLaunchKernelGGL((oz_igemm_big_kernel<false>), dim3(G), dim3(JT), 0, s, tab, m, nmod, kc,
+                         map, m0, levels);
     return hipGetLastError();
   }
   TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);

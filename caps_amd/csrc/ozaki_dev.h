@@ -151,12 +151,37 @@ __device__ __forceinline__ double i128_to_double(__int128 x) {
   return neg ? -r : r;
 }
 
+// s + e == a + b exactly (Knuth's TwoSum; no ordering of a and b needed)
+__device__ __forceinline__ double two_sum(double a, double b, double &e) {
+  const double s = a + b;
+  const double bv = s - a;
+  e = (a - (s - bv)) + (b - bv);
+  return s;
+}
+
+// a + b rounded to odd: the sum itself when it is a double, else the one of
+// its two neighbouring doubles whose last significand bit is 1 (a + b != 0
+// whenever the rounded sum is inexact, so the step below never starts at 0)
+__device__ __forceinline__ double add_round_odd(double a, double b) {
+  double e;
+  const double s = two_sum(a, b, e);
+  long long bits = __double_as_longlong(s);
+  if (e != 0.0 && !(bits & 1)) bits += ((e > 0.0) == (s > 0.0)) ? 1 : -1;
+  return __longlong_as_double(bits);
+}
+
 // X = sum_p S_p 2^(37p) reduced to (-M/2, M/2): the quotient k = round(S/M)
 // from an FP64 estimate (|S| < 2^14.8 M, so the estimate of S/M is within
 // 2^-36 of it: exact unless X were within 2^-36 M of M/2, which the scaling
 // bound, |X| <= (1/2 - 2^-31) M, excludes), D_p = S_p - k M_p exact (|k| <
-// 2^15, |D_p| < 2^53), X assembled exactly in 128 bits (|X| < M/2 <= 2^127)
-// and rounded once to FP64
+// 2^15, |D_p| < 2^53), and X = sum_p D_p 2^(37p) rounded once to FP64:
+//  * two pieces: D_1 2^37 and D_0 are doubles, one rounded addition;
+//  * three pieces: the correctly rounded sum of the three doubles a = D_2
+//    2^74, b = D_1 2^37, c = D_0 by rounding to odd (Boldo and Melquiond,
+//    "Emulation of FMA and correctly rounded sums: proved algorithms using
+//    rounding to odd", IEEE Trans. Computers 57(4), 2008): (uh, ul) =
+//    TwoSum(b, c), (th, tl) = TwoSum(a, uh), v = RO(tl + ul), X = RN(th + v);
+//  * four pieces: X assembled exactly in 128 bits (|X| < M/2 <= 2^127).
 template <int NP>
 __device__ __forceinline__ double crt_finish(const double (&S)[NP],
                                              const OzakiConsts *__restrict__ kc) {
@@ -164,11 +189,24 @@ __device__ __forceinline__ double crt_finish(const double (&S)[NP],
 #pragma unroll
   for (int p = NP - 2; p >= 0; p--) approx = fma(approx, 0x1p37, S[p]);
   const double k = rint(approx * kc->invM);
-  __int128 X = 0;
+  double D[NP];
 #pragma unroll
-  for (int p = NP - 1; p >= 0; p--)
-    X = X * ((__int128)1 << OZ_PIECE_BITS) + (__int128)(long long)fma(-k, kc->Mp[p], S[p]);
-  return i128_to_double(X);
+  for (int p = 0; p < NP; p++) D[p] = fma(-k, kc->Mp[p], S[p]);
+  if constexpr (NP == 1) {
+    return D[0];
+  } else if constexpr (NP == 2) {
+    return fma(D[1], 0x1p37, D[0]);  // D_1 2^37 exact: one rounding
+  } else if constexpr (NP == 3) {
+    double ul, tl;
+    const double uh = two_sum(D[1] * 0x1p37, D[0], ul);
+    const double th = two_sum(D[2] * 0x1p74, uh, tl);
+    return th + add_round_odd(tl, ul);
+  } else {
+    __int128 X = 0;
+#pragma unroll
+    for (int p = NP - 1; p >= 0; p--) X = X * ((__int128)1 << OZ_PIECE_BITS) + (__int128)(long long)D[p];
+    return i128_to_double(X);
+  }
 }
 
 // offset of element (i, j) of an n x n matrix in the CAPS layout (nproc = 1):
