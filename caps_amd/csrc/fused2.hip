@@ -29,6 +29,7 @@ constexpr int T = 256;
 // global-memory views (accesses through them are global_*, not flat_*)
 typedef __attribute__((address_space(1))) unsigned short gushort;
 typedef const __attribute__((address_space(1))) unsigned gcuint;
+typedef __attribute__((address_space(1))) unsigned guint;
 typedef __attribute__((address_space(1))) double gdouble;
 typedef const __attribute__((address_space(1))) int gcint;
 
@@ -143,28 +144,28 @@ __global__ __launch_bounds__(T) void bb_post_f64_kernel(const BBPost *__restrict
 }
 
 // ---------------------------------------------------------------- residue pre
-// thread: 2 consecutive positions (2-byte accesses); grid.y: modulus; one
-// instantiation per side.  Depth-2 values are sums of at most 16 residues:
-// |x| <= 2048, reduced by mod_k.  Offsets within a plane are 32-bit (q < 2^31).
+// thread: 4 consecutive positions (one 4-byte word per operand); grid.y:
+// modulus; one instantiation per side.  Depth-2 values are sums of at most 16
+// residues: |x| <= 2048, reduced by mod_k.  Offsets within a plane are 32-bit
+// (q < 2^31).
 template <int SIDE>
 __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__ P, long long q,
                                                        const OzakiConsts *__restrict__ kc) {
   const int l = blockIdx.y;
   const int mi = kc->m[l], c22 = kc->c22[l];
-  const int8_t *in = reinterpret_cast<const int8_t *>(P->in[SIDE]) + l * P->sin[SIDE];
+  const char *in = reinterpret_cast<const char *>(P->in[SIDE]) + l * P->sin[SIDE];
   char *obase = reinterpret_cast<char *>(P->out[SIDE]) + l * P->sout[SIDE];
   const long long step = P->step;
-  const unsigned qq = (unsigned)q, n2 = (unsigned)(q >> 1);
-  for (unsigned u = blockIdx.x * T + threadIdx.x; u < n2; u += gridDim.x * T) {
+  const unsigned qq = (unsigned)q, n4 = (unsigned)(q >> 2);
+  for (unsigned u = blockIdx.x * T + threadIdx.x; u < n4; u += gridDim.x * T) {
     unsigned w[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-      for (int b = 0; b < 4; b++)
-        w[a][b] = *reinterpret_cast<const unsigned short *>(in + (4 * a + b) * qq + 2 * u);
-    int l1[2][7][4];  // [position][level-1 operand i][sub-position b]
+      for (int b = 0; b < 4; b++) w[a][b] = *(gcuint *)(in + (4 * a + b) * qq + 4 * u);
+    int l1[4][7][4];  // [position][level-1 operand i][sub-position b]
 #pragma unroll
-    for (int h = 0; h < 2; h++)
+    for (int h = 0; h < 4; h++)
 #pragma unroll
       for (int b = 0; b < 4; b++) {
         int qv[4], o[7];
@@ -177,15 +178,15 @@ __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__
     int k = 0;  // index among the written operands (compile-time after unrolling)
 #pragma unroll
     for (int i = 0; i < 7; i++) {
-      int o0[7], o1[7];
-      level_ops(SIDE, l1[0][i], o0);
-      level_ops(SIDE, l1[1][i], o1);
+      int o[4][7];
+#pragma unroll
+      for (int h = 0; h < 4; h++) level_ops(SIDE, l1[h][i], o[h]);
 #pragma unroll
       for (int j = 0; j < 7; j++) {
         if (bb_plain(SIDE, i) && bb_plain(SIDE, j)) continue;
-        const unsigned v = (unsigned)(mod_k(o0[j], c22, mi) & 0xff) |
-                           ((unsigned)(mod_k(o1[j], c22, mi) & 0xff) << 8);
-        *reinterpret_cast<unsigned short *>(obase + k * step + 2 * u) = (unsigned short)v;
+        const unsigned v = pack4_lo(mod_k(o[0][j], c22, mi), mod_k(o[1][j], c22, mi),
+                                    mod_k(o[2][j], c22, mi), mod_k(o[3][j], c22, mi));
+        *(guint *)(obase + k * step + 4 * u) = v;
         k++;
       }
     }
@@ -288,9 +289,7 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
     unsigned w[49];
 #pragma unroll
     for (int k = 0; k < 49; k++) w[k] = *(gcuint *)(qb + k * step);
-    unsigned out[16];
-#pragma unroll
-    for (int o = 0; o < 16; o++) out[o] = 0;
+    int res[16][4];  // [sub-block of C][position]
 #pragma unroll
     for (int byte = 0; byte < 4; byte++) {
       int l1[7][4];
@@ -310,12 +309,11 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
         for (int i = 0; i < 7; i++) qv[i] = l1[i][bb];
         Ew<EW_BFS_POST>::apply(qv, c);  // |.| <= 2048
 #pragma unroll
-        for (int aa = 0; aa < 4; aa++)
-          out[4 * aa + bb] |= (unsigned)(mod_k(c[aa], c22, mi) & 0xff) << (8 * byte);
+        for (int aa = 0; aa < 4; aa++) res[4 * aa + bb][byte] = mod_k(c[aa], c22, mi);
       }
     }
 #pragma unroll
-    for (int o = 0; o < 16; o++) R[l][o][g] = out[o];
+    for (int o = 0; o < 16; o++) R[l][o][g] = pack4_lo(res[o][0], res[o][1], res[o][2], res[o][3]);
   }
   __syncthreads();
 
@@ -366,7 +364,7 @@ hipError_t launch_bb_pre(const BBPre *P, long long q, bool residue, int nmod,
     hipLaunchKernelGGL(bb_pre_f64_kernel, g, dim3(T), 0, s, P, q);
   } else {
     if (q % 4 || q >= (1LL << 31)) return hipErrorInvalidValue;
-    dim3 g(blocks_for(q / 2, 2 * nmod), nmod);
+    dim3 g(blocks_for(q / 4, 2 * nmod), nmod);
     hipLaunchKernelGGL(bb_pre_res_kernel<0>, g, dim3(T), 0, s, P, q, kc);
     if (!a_only) hipLaunchKernelGGL(bb_pre_res_kernel<1>, g, dim3(T), 0, s, P, q, kc);
   }

@@ -73,6 +73,17 @@ __device__ __forceinline__ int mod_k(int x, int c22, int mi) {
   return __mul24(q, -mi) + x;
 }
 
+// the low bytes of four ints as one word, byte h from b_h.  v_perm_b32
+// (bytes 0-3 of the second operand are selectors 0-3, of the first 4-7, 0x0c
+// gives 0): written this way the compiler keeps the byte shifts out of the
+// mad24 arithmetic that produced the values (folding them turns mad24 into
+// 32-bit multiplies).
+__device__ __forceinline__ unsigned pack4_lo(int b0, int b1, int b2, int b3) {
+  const unsigned lo = __builtin_amdgcn_perm((unsigned)b1, (unsigned)b0, 0x0c0c0400u);
+  const unsigned hi = __builtin_amdgcn_perm((unsigned)b3, (unsigned)b2, 0x04000c0cu);
+  return lo | hi;
+}
+
 // symmetric residue of the FP64 integer x (|x| <= 2^63) modulo mi: the FP64
 // quotient estimate x * inv is within |x / mi| 2^-52 <= 11 of x / mi, so r =
 // x - q mi is an integer with |r| < 12 mi <= 3072, exact in one fma; mod_k
