@@ -198,6 +198,7 @@ constexpr int ISTR = IBK + 16;  // LDS row stride (bytes): b128 reads of 16 lane
 // slice origin; plain operands only.
 typedef __attribute__((address_space(1))) unsigned gu32;  // stores as global_*, not flat_*
 typedef __attribute__((address_space(1))) v4i gv4i;
+typedef const __attribute__((address_space(1))) v4u gv4u;
 
 template <int CHECK>
 __device__ __forceinline__ void gload_i8(const int8_t *__restrict__ Ab,
@@ -472,7 +473,7 @@ struct BigTile {
   const int8_t *Ap, *Bp;
   int8_t *Cp;
   __amdgpu_buffer_rsrc_t ra, rb;  // buffer descriptors of the operand planes
-  int mi, c14, row0, col0, ti, tj;
+  int mi, c14, c22, row0, col0, ti, tj;
   float mf, invf;
 };
 
@@ -501,6 +502,7 @@ __device__ __forceinline__ BigTile big_tile(const RGemm *__restrict__ tab, const
   t.mf = (float)t.mi;
   t.invf = 1.0f / t.mf;
   t.c14 = kc->c14[l];
+  t.c22 = kc->c22[l];
   t.row0 = ti * JBM;
   t.col0 = tj * JBM;
   t.ti = ti;
@@ -851,6 +853,184 @@ __global__ __launch_bounds__(T256) void oz_bgemm_kernel(const RGemm *__restrict_
   }
 }
 
+// ------------------------------------------------------------------ BF16 GEMM, 256 x 256 tiles
+// The BF16 variant with the large kernel's tiling: 256 x 256 tiles, 8 waves
+// of 128 x 64 (2 per SIMD), K-steps of 64 on v_mfma_f32_16x16x32_bf16 with
+// FP32 accumulation folded mod m every BFOLD k as above.  A K-step's INT8
+// slabs (256 x 64 bytes of A and of B) come into registers (4 x 16 bytes per
+// lane), are widened once to BF16 (cvt_f32 of a sign-extended byte, then
+// cvt_pk_bf16: 1.5 VALU per value) and stored to the other LDS stage after
+// the step's MFMAs.  A BF16 LDS row is 64 values = 128 bytes, 8 chunks of 16
+// with the INT8 kernel's swizzle, so the fragment reads are the INT8
+// kernel's.  C leaves through LDS as in the INT8 kernel.  Same results, bit
+// for bit, as the INT8 kernels.  Needs m0 % 256 == 0.
+constexpr int KBK = 64;                   // K per step (BF16 values)
+constexpr int KSTAGE = JBM * KBK * 2;     // 32 KB per operand per stage
+
+__device__ __forceinline__ void widen16_bf16(v4u w, v4u &lo, v4u &hi) {
+  typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+  unsigned o[8];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const float f0 = (float)(int)(signed char)(w[q] >> (16 * h));
+      const float f1 = (float)(int)(signed char)(w[q] >> (16 * h + 8));
+      o[2 * q + h] = __builtin_bit_cast(unsigned, v2bf{(__bf16)f0, (__bf16)f1});
+    }
+  lo = v4u{o[0], o[1], o[2], o[3]};
+  hi = v4u{o[4], o[5], o[6], o[7]};
+}
+
+__global__ __launch_bounds__(JT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+oz_bgemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
+                    const OzakiConsts *__restrict__ kc, TileMap map, int m0, int d) {
+  __shared__ __attribute__((aligned(1024))) int8_t A0[KSTAGE];
+  __shared__ __attribute__((aligned(1024))) int8_t A1[KSTAGE];
+  __shared__ __attribute__((aligned(1024))) int8_t B0[KSTAGE];
+  __shared__ __attribute__((aligned(1024))) int8_t B1[KSTAGE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves: rows wm * 128, columns wn * 64
+  const int l16 = lane & 15, lk = lane >> 4;
+  const int G = gridDim.x;
+  const int nk = m / KBK;
+  const int fold_every = BFOLD / KBK;
+  // fragment reads: row l16, chunk lk (k = 8 lk .. + 7) and lk + 4 (k + 32)
+  const int s16 = jswz(l16);
+  const int rd0 = l16 * 128 + ((lk ^ s16) << 4);
+  const int rd1 = l16 * 128 + (((lk + 4) ^ s16) << 4);
+  const int aoff = wm * 128 * 128, boff = wn * 64 * 128;
+  const long long mm0 = (long long)m0 * m0;
+  // staging: lane chunk idx = tid + 512 it: row idx >> 2, 16 INT8 k at 16 (idx & 3)
+  int srow[2], skq[2], sdst[2][2];
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const int idx = tid + JT * it;
+    srow[it] = idx >> 2;
+    skq[it] = idx & 3;
+    const int sw = jswz(srow[it]);
+    sdst[it][0] = srow[it] * 128 + (((2 * skq[it]) ^ sw) << 4);
+    sdst[it][1] = srow[it] * 128 + (((2 * skq[it] + 1) ^ sw) << 4);
+  }
+  auto a_off = [&](int row, int k) -> long long {
+    return d == 0 ? (long long)row * m0 + k
+                  : quad_leaf(row / m0, k / m0, d) * mm0 + (long long)(row % m0) * m0 + (k % m0);
+  };
+  auto b_off = [&](int col, int k) -> long long {
+    return d == 0 ? (long long)col * m0 + k
+                  : quad_leaf(k / m0, col / m0, d) * mm0 + (long long)(col % m0) * m0 + (k % m0);
+  };
+  auto gload = [&](const BigTile &c, int k, v4u (&ga)[2], v4u (&gb)[2]) {
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      ga[it] = *(const gv4u *)(c.Ap + a_off(c.row0 + srow[it], k + 16 * skq[it]));
+      gb[it] = *(const gv4u *)(c.Bp + b_off(c.col0 + srow[it], k + 16 * skq[it]));
+    }
+  };
+  auto lstore = [&](const v4u (&ga)[2], const v4u (&gb)[2], int8_t *As, int8_t *Bs) {
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      v4u lo, hi;
+      widen16_bf16(ga[it], lo, hi);
+      *reinterpret_cast<v4u *>(As + sdst[it][0]) = lo;
+      *reinterpret_cast<v4u *>(As + sdst[it][1]) = hi;
+      widen16_bf16(gb[it], lo, hi);
+      *reinterpret_cast<v4u *>(Bs + sdst[it][0]) = lo;
+      *reinterpret_cast<v4u *>(Bs + sdst[it][1]) = hi;
+    }
+  };
+
+  int tile = xcd_tile(blockIdx.x, G);
+  if (tile >= map.total) return;
+  BigTile t = big_tile(tab, map, tile, nmod, kc, (long long)m * m);
+  v4u ga[2], gb[2];
+  gload(t, 0, ga, gb);
+  lstore(ga, gb, A0, B0);
+  __syncthreads();
+
+  for (int round = 0;; round++) {
+    v4f acc[8][4];
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[r][c] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < nk; kt++) {
+      const bool odd = kt & 1;
+      if (kt + 1 < nk) gload(t, (kt + 1) * KBK, ga, gb);
+      const int8_t *Ab = (odd ? A1 : A0) + aoff;
+      const int8_t *Bb = (odd ? B1 : B0) + boff;
+      v8bf a0[8], b0[4], a1[8], b1[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) b0[c] = *reinterpret_cast<const v8bf *>(Bb + rd0 + c * 16 * 128);
+#pragma unroll
+      for (int r = 0; r < 8; r++) a0[r] = *reinterpret_cast<const v8bf *>(Ab + rd0 + r * 16 * 128);
+#pragma unroll
+      for (int c = 0; c < 4; c++) b1[c] = *reinterpret_cast<const v8bf *>(Bb + rd1 + c * 16 * 128);
+#pragma unroll
+      for (int r = 0; r < 8; r++) a1[r] = *reinterpret_cast<const v8bf *>(Ab + rd1 + r * 16 * 128);
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[r], b0[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[r], b1[c], acc[r][c], 0, 0, 0);
+      if ((kt + 1) % fold_every == 0) {
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[r][c][q] = fold_f32(acc[r][c][q], t.mf, t.invf);
+      }
+      if (kt + 1 < nk) lstore(ga, gb, odd ? A0 : A1, odd ? B0 : B1);
+      __syncthreads();
+    }
+    // the next tile's first stage goes to A0 / B0 (free: nk is even, the last
+    // step read A1 / B1) before this tile's epilogue
+    const int next = (round + 1) * G + xcd_tile(blockIdx.x, G);
+    const bool has_next = next < map.total;
+    BigTile tn = t;
+    if (has_next) {
+      tn = big_tile(tab, map, next, nmod, kc, (long long)m * m);
+      gload(tn, 0, ga, gb);
+    }
+    // C through LDS (A1: columns 0..127, B1: 128..255, as in the INT8 kernel)
+    const int c22 = t.c22;
+    unsigned *Ch = reinterpret_cast<unsigned *>(wn < 2 ? A1 : B1);
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        int res[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          res[q] = mod_k((int)fold_f32(acc[r][c][q], t.mf, t.invf), c22, t.mi);  // |.| <= 1.5 m
+        const int col = (wn & 1) * 64 + c * 16 + l16;
+        const int w = wm * 32 + r * 4 + lk;
+        Ch[col * 64 + (w ^ (4 * l16))] = pack4_lo(res[0], res[1], res[2], res[3]);
+      }
+    if (has_next) lstore(ga, gb, A0, B0);
+    __syncthreads();
+    const int j16 = tid & 15;
+    int8_t *Cl = t.Cp + quad_leaf(t.row0 / m0, t.col0 / m0, d) * mm0 +
+                 (long long)(t.col0 % m0 + (tid >> 4)) * m0 + (t.row0 % m0) + 16 * j16;
+#pragma unroll
+    for (int pass = 0; pass < 8; pass++) {
+      const int ccl = (pass * 32 + (tid >> 4)) & 127;  // pass < 4: A1, else B1
+      const unsigned *Cs = reinterpret_cast<const unsigned *>(pass < 4 ? A1 : B1);
+      const v4i v = *reinterpret_cast<const v4i *>(Cs + ccl * 64 + ((4 * j16) ^ (4 * (ccl & 15))));
+      *(gv4i *)(Cl + pass * 32 * m0) = v;
+    }
+    if (!has_next) break;
+    t = tn;
+    __syncthreads();  // the C reads of A1 / B1 end before the next tile's K-step 0 fills them
+  }
+}
+
 // ------------------------------------------------------------------ reconstruct
 template <int NP, bool VEC>
 __global__ __launch_bounds__(T256) void oz_recon_kernel(const OzLeaf *__restrict__ tab, int m,
@@ -918,6 +1098,15 @@ hipError_t launch_residue_gemm_bf16(const RGemm *tab, int count, int m, int leve
   const int m0 = m >> levels;
   if (levels < 0 || (m0 << levels) != m) return hipErrorInvalidValue;
   bool ok;
+  if (m0 % JBM == 0) {
+    // large tiles: one block (8 waves, 128 KB of LDS) per CU
+    TileMap map = make_tilemap(m, JBM, (long long)count * nmod, ok);
+    if (!ok) return hipErrorInvalidValue;
+    const int G = persistent_grid(1, device_cu_count(), map.total);
+    hipLaunchKernelGGL(oz_bgemm_big_kernel, dim3(G), dim3(JT), 0, s, tab, m, nmod, kc, map, m0,
+                       levels);
+    return hipGetLastError();
+  }
   TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);
   if (!ok) return hipErrorInvalidValue;
   const int G = persistent_grid(2, device_cu_count(), map.total);
