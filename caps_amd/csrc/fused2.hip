@@ -186,7 +186,7 @@ __global__ __launch_bounds__(T) void bb_pre_res_kernel(const BBPre *__restrict__
         if (bb_plain(SIDE, i) && bb_plain(SIDE, j)) continue;
         const unsigned v = pack4_lo(mod_k(o[0][j], c22, mi), mod_k(o[1][j], c22, mi),
                                     mod_k(o[2][j], c22, mi), mod_k(o[3][j], c22, mi));
-        *(guint *)(obase + k * step + 4 * u) = v;
+        __builtin_nontemporal_store(v, (guint *)(obase + k * step + 4 * u));
         k++;
       }
     }
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restric
           const int r0 = plain ? o0[j] : mod_k(o0[j], c22, mi);
           const int r1 = plain ? o1[j] : mod_k(o1[j], c22, mi);
           const unsigned v = (unsigned)(r0 & 0xff) | ((unsigned)(r1 & 0xff) << 8);
-          ((gushort *)(ob + (7 * i + j) * q))[u] = (unsigned short)v;
+          __builtin_nontemporal_store((unsigned short)v, (gushort *)(ob + (7 * i + j) * q) + u);
         }
       }
     }
@@ -341,8 +341,8 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
     const int gi = (aa & 1) * 2 * sub + (bb & 1) * sub + ri;
     const int gj = (aa >> 1) * 2 * sub + (bb >> 1) * sub + cj;
     const double X = crt_finish<NP>(S, kc);
-    ((gdouble *)P->out_c)[o * q + dst_in_sub] =
-        ldexp(X, -(((gcint *)P->sa)[gi] + ((gcint *)P->sb)[gj]));
+    __builtin_nontemporal_store(ldexp(X, -(((gcint *)P->sa)[gi] + ((gcint *)P->sb)[gj])),
+                                (gdouble *)P->out_c + o * q + dst_in_sub);
   }
 }
 

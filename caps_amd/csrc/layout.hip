@@ -4,7 +4,8 @@
 // levels of quadrants in the order 11, 21, 12, 22, then the m x m leaf split
 // into bs x bs blocks, column-major between blocks and within a block.
 //
-// HBM-bound (16 bytes per element moved).  The tile kernels stage a 64 x 64
+// HBM-bound (16 bytes per element moved; stores are non-temporal: the
+// result is not read again by this pass).  The tile kernels stage a 64 x 64
 // tile of a leaf through LDS: the column-major side moves as 64 columns of 64
 // contiguous doubles, and the CAPS side as the tile's runs in storage order:
 // with bs dividing 64 the tile covers 64 / bs block columns, each one
@@ -96,7 +97,7 @@ __device__ __forceinline__ void layout_tile(const double *__restrict__ src,
     for (int q = threadIdx.x; q < TS * TS; q += LT) {
       int r, c;
       const long long o = caps_at(q, r, c);
-      dst[leaf + o] = S[c][r];
+      __builtin_nontemporal_store(S[c][r], dst + leaf + o);
     }
   } else {
 #pragma unroll 4
@@ -107,7 +108,8 @@ __device__ __forceinline__ void layout_tile(const double *__restrict__ src,
     }
     __syncthreads();
 #pragma unroll 4
-    for (int c = grp; c < TS; c += LT / 64) dst[(long long)(j0 + c) * n + i0 + lane] = S[c][lane];
+    for (int c = grp; c < TS; c += LT / 64)
+      __builtin_nontemporal_store(S[c][lane], dst + (long long)(j0 + c) * n + i0 + lane);
   }
   __syncthreads();  // S is reused by the block's next tile
 }
@@ -159,12 +161,17 @@ unsigned grid_for(long long total, int per_problem) {
 // or is a multiple of it
 bool tiled(int m, int bs) { return m % TS == 0 && (TS % bs == 0 || bs % TS == 0); }
 
+// one tile per workgroup (up to 65,535 per matrix), two when writing small
+// blocks (measured at n = 8192: 5.8-6.0 TB/s for bs = 1 and for bs = 8 out of
+// the CAPS layout, 5.3 for bs = 8 into it, against 5.1-5.4 with 8 persistent
+// workgroups per CU)
 template <bool TO_CAPS>
 hipError_t launch_tiles(const ReorderPtrs *tab, int count, const double *src, double *dst,
                         int n, int nrec, int bs, hipStream_t s) {
   const long long tiles = (long long)(n / TS) * (n / TS);
-  long long g = (long long)device_cu_count() * 8 / (count > 0 ? count : 1);
-  if (g > tiles) g = tiles;
+  const long long per = (TO_CAPS && bs > 1 && bs < TS) ? 2 : 1;
+  long long g = (tiles + per - 1) / per;
+  if (g > 65535) g = 65535;
   if (g < 1) g = 1;
   dim3 grid((unsigned)g, (unsigned)count);
 #define CAPS_LAYOUT_TILE(B)                                                                    \
