@@ -443,6 +443,7 @@ class MLP:
     forward, backward and Adam kernels run on the GPU (caps_mlp_* in the C ABI)."""
 
     N_IN, N_OUT = 8, 2
+    _installed = None
 
     def __init__(self, seed: int = 0):
         self._h = ctypes.c_void_p()
@@ -495,10 +496,12 @@ class MLP:
         norm = np.ascontiguousarray(norm, dtype=np.float32)
         self._norm = norm
         _check(load().caps_planner_use_model(self._h, norm.ctypes.data, norm.size))
+        MLP._installed = self  # installed models stay alive until cleared
 
     @staticmethod
     def clear_planner() -> None:
         _check(load().caps_planner_use_model(None, None, 0))
+        MLP._installed = None
 
     def train(self, X, T, M, epochs=200, batch=256, lr=1e-3, seed=0) -> float:
         X, T, M = self._f32(X, self.N_IN), self._f32(T, self.N_OUT), self._f32(M, self.N_OUT)

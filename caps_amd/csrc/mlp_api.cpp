@@ -14,6 +14,7 @@
 
 #include "../../include/caps_amd.h"
 #include "mlp.h"
+#include "planner.h"
 
 namespace caps {
 int set_error(int code, const char *msg);
@@ -128,6 +129,7 @@ int caps_mlp_create(caps_mlp **out, unsigned long long seed) {
 
 int caps_mlp_destroy(caps_mlp *h) {
   if (!h) return CAPS_OK;
+  caps::planner_forget_model(h);
   hipStreamSynchronize(h->s);
   for (float *q : {h->p, h->m1, h->m2, h->g, h->X, h->T, h->M, h->Y, h->A1, h->A2, h->partial,
                    h->loss})

@@ -25,4 +25,7 @@ int choose_mode(const caps_desc &d);
 bool mlp_planner_choose(const caps_desc &d, int mask, double max_err, double spread, int &mode,
                         int &moduli, int &cutoff, std::string &pat, double *pred = nullptr);
 
+// Uninstall m if it is the installed planner model (caps_mlp_destroy calls it).
+void planner_forget_model(const caps_mlp *m);
+
 }  // namespace caps

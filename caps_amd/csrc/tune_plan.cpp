@@ -126,6 +126,14 @@ ModelSlot &model() {
 
 }  // namespace
 
+// caps_mlp_destroy of the installed planner model uninstalls it first (the
+// library does not own the model; a destroyed one must not stay installed)
+void planner_forget_model(const caps_mlp *m) {
+  ModelSlot &ms = model();
+  std::lock_guard<std::mutex> g(ms.mu);
+  if (ms.m == m) ms.m = nullptr;
+}
+
 bool mlp_planner_choose(const caps_desc &d, int mask, double max_err, double spread, int &mode,
                         int &moduli, int &cutoff, std::string &pat, double *pred) {
   ModelSlot &ms = model();

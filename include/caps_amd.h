@@ -312,7 +312,8 @@ int caps_mlp_train(caps_mlp *m, const float *X, const float *T, const float *M, 
                    float *final_loss);
 /* Install m (NULL: remove) as the planner of CAPS_MODE_AUTO calls that have
  * no tuning-table entry.  norm: feature means[8], feature scales[8], target
- * means[2], target scales[2]; targets are log2(ms) and log10(error). */
+ * means[2], target scales[2]; targets are log2(ms) and log10(error).  The
+ * caller keeps ownership of m; caps_mlp_destroy(m) uninstalls it. */
 int caps_planner_use_model(caps_mlp *m, const float *norm, int nnorm);
 
 #ifdef __cplusplus

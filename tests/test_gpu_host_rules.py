@@ -100,3 +100,19 @@ def test_small_tile_gives_the_same_bits(mode, n, r, pat, cutoff):
         out.append(C)
     assert torch.equal(out[0], out[1])
     caps_amd.release()
+
+
+def test_destroyed_planner_model_is_uninstalled():
+    """The library does not own an installed planner model: destroying it
+    uninstalls it, and later CAPS_MODE_AUTO choices fall back to the rule
+    without a model instead of reading freed memory."""
+    d = caps_amd.MatDescriptor.one_device(2048, 2, 1)
+    m = caps_amd.MLP(seed=1)
+    norm = np.concatenate([np.zeros(8), np.ones(8), np.zeros(2), np.ones(2)]).astype(np.float32)
+    m.use_as_planner(norm)
+    print("[gpu] planner model installed, then destroyed", flush=True)
+    caps_amd.plan_choice(d, caps_amd.MODE_AUTO)  # consults the installed model
+    m.close()
+    caps_amd.MLP._installed = None
+    c = caps_amd.plan_choice(d, caps_amd.MODE_AUTO)
+    assert c["predicted_ms"] < 0
