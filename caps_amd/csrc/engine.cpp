@@ -506,13 +506,19 @@ struct Builder {
       M.rc = reinterpret_cast<int8_t *>(RC.p);
       const size_t mt = add_table(&M, sizeof(M));
       plan.crt_host.push_back(M);
-      const size_t hi = plan.crt_host.size() - 1;
-      push(STEP_CRT_SCALE, moduli, 1, n, d.bs, mt);
-      steps().back().aux = hi;
-      push(STEP_CRT_SPLIT, moduli, 1, n, d.bs, mt);
-      steps().back().aux = hi;
       top_batch.push_back({RA, RB, RC});
       mats.push_back(mt);
+    }
+    // every item's scale, then every item's split (the items are independent;
+    // consecutive steps of one kind can share a segmented launch)
+    const size_t h0 = plan.crt_host.size() - items.size();
+    for (size_t i = 0; i < items.size(); i++) {
+      push(STEP_CRT_SCALE, moduli, 1, n, d.bs, mats[i]);
+      steps().back().aux = h0 + i;
+    }
+    for (size_t i = 0; i < items.size(); i++) {
+      push(STEP_CRT_SPLIT, moduli, 1, n, d.bs, mats[i]);
+      steps().back().aux = h0 + i;
     }
     if (top2) {
       bfs2(top_batch[0], n, d.nrec, 0, &plan.crt_host.back());
@@ -808,13 +814,24 @@ void merge_groups(Plan &plan, const std::vector<std::vector<Step>> &gsteps, int 
       }
     }
     if (best < 0) return;
-    if (keys[best].kind < 0 || best_n < 2) {
+    // a run of per-item CRT steps in one group merges on its own too
+    const bool run = keys[best].kind == STEP_MULTI_CRT &&
+                     pos[best] + 1 < gsteps[best].size() &&
+                     merge_key(plan, gsteps[best][pos[best] + 1]) == keys[best];
+    if (keys[best].kind < 0 || (best_n < 2 && !run)) {
       plan.steps.push_back(gsteps[best][pos[best]++]);
       continue;
     }
     std::vector<const Step *> heads;
     for (size_t g = 0; g < G; g++)
-      if (pos[g] < gsteps[g].size() && keys[g] == keys[best]) heads.push_back(&gsteps[g][pos[g]++]);
+      if (pos[g] < gsteps[g].size() && keys[g] == keys[best]) {
+        heads.push_back(&gsteps[g][pos[g]++]);
+        // per-item scale / split / reconstruct steps of one group are
+        // independent of each other: a run of them joins the same launch
+        if (keys[best].kind == STEP_MULTI_CRT)
+          while (pos[g] < gsteps[g].size() && merge_key(plan, gsteps[g][pos[g]]) == keys[best])
+            heads.push_back(&gsteps[g][pos[g]++]);
+      }
     emit_merged(plan, keys[best], heads, moduli);
   }
 }
@@ -883,7 +900,7 @@ int build_batch_plan(Plan &plan, const std::vector<BatchItem> &items, int mode, 
     sum += peaks[g];
     mx = std::max(mx, peaks[g]);
   }
-  const bool merge = groups.size() > 1 && sum <= merge_budget;
+  const bool merge = (groups.size() > 1 || groups[0].size() > 1) && sum <= merge_budget;
   char *base = reinterpret_cast<char *>(work_base);
   if (merge) {
     std::vector<std::vector<Step>> gsteps(groups.size());
