@@ -677,10 +677,14 @@ MergeKey merge_key(const Plan &plan, const Step &st) {
       k = {STEP_MULTI_GEMM, dgemm_block_tile((int)st.n, st.levels, st.tile), 0};
       break;
     case STEP_OZ_GEMM: {
-      // INT8 products that would take the 128 x 128 kernel alone; those with
-      // 256-tile leaves keep their own launch
+      // INT8 products that would take the 128 x 128 kernel alone; those on
+      // 256-tile leaves keep their own launch, and so do those that take the
+      // 256 tiles with partial edges when they fill the device (256 tiles)
       const int m = (int)st.n, m0 = m >> st.levels;
-      const bool big = st.tile == 0 && m0 % 256 == 0 && (long long)m * m < (1LL << 31);
+      const long long t256 =
+          (long long)((m + 255) / 256) * ((m + 255) / 256) * st.count * st.op;  // op: moduli
+      const bool big = (st.tile == 0 && m0 % 256 == 0 && (long long)m * m < (1LL << 31)) ||
+                       (big_edge_gemm(m, st.levels, st.tile) && t256 >= 256);
       if (st.bs == 0 && !big && residue_gemm_check(m, st.levels) >= 0)
         k = {STEP_MULTI_RGEMM, 0, 0};
       break;

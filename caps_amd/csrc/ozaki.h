@@ -97,6 +97,9 @@ hipError_t launch_residue_gemm_bf16(const RGemm *d_tab, int count, int m, int le
 // tiles per dimension, gy = leaves x moduli, aux = quadtree levels | check
 // mode << 8, the mode residue_gemm_check gives; -1: not supported).
 int residue_gemm_check(int m, int levels);
+// launch_residue_gemm takes the large tiles with partial edges for this
+// (plain leaves, m % 16 == 0, little padding)
+bool big_edge_gemm(int m, int levels, int tile);
 hipError_t launch_residue_gemm_multi(const char *d_tables, const VSeg *d_segs, int nseg,
                                      int total_tiles, int nmod, const OzakiConsts *d_consts,
                                      hipStream_t s);

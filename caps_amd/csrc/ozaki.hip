@@ -460,6 +460,29 @@ __device__ __forceinline__ void stage_load(__amdgpu_buffer_rsrc_t ra, __amdgpu_b
   }
 }
 
+// stage_load for plain m x m operands (d = 0) of any m % 16 == 0: rows /
+// columns at or beyond m and 16-byte K chunks at or beyond m read as zero
+// (their buffer offset is the plane size, past num_records: the buffer range
+// check returns zeros without a memory access)
+__device__ __forceinline__ void stage_load_edge(__amdgpu_buffer_rsrc_t ra,
+                                                __amdgpu_buffer_rsrc_t rb, int row0, int col0,
+                                                int k, int m, int8_t *As, int8_t *Bs, int wave,
+                                                int lane) {
+  const int rl = lane >> 3;
+  const int c = (lane & 7) ^ ((((wave & 1) << 1) | (rl >> 2)) << 1);
+  const int oob = m * m;
+  const bool kin = k + c * 16 < m;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int r = j * 64 + wave * 8 + rl;
+    const int slot0 = (j * 8 + wave) * 64;
+    const int ao = (kin && row0 + r < m) ? (row0 + r) * m + k + c * 16 : oob;
+    const int bo = (kin && col0 + r < m) ? (col0 + r) * m + k + c * 16 : oob;
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_ptr)(As + slot0 * 16), 16, ao, 0, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_ptr)(Bs + slot0 * 16), 16, bo, 0, 0, 0);
+  }
+}
+
 // block barrier for LDS stages filled by this wave's own loads: the caller
 // has waited for them (vmcnt) and consumed every LDS read of the stage it
 // leaves; the empty asm statements keep memory operations on their side
@@ -515,7 +538,10 @@ __device__ __forceinline__ BigTile big_tile(const RGemm *__restrict__ tab, const
 // SMALL_K: K = m <= 16384, so |sums| <= m 2^14 < 2^28 and the one-pass
 // exact reduction res_byte128 applies (sums start at 128); larger K takes the
 // general one
-template <bool SMALL_K>
+// EDGE: plain operands (d = 0) whose size m is a multiple of 16 but not of
+// 256: the last tile row / column and K-step are partial, their missing
+// elements read as zero and the stores beyond m are skipped.
+template <bool SMALL_K, bool EDGE>
 __global__ __launch_bounds__(JT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
                     const OzakiConsts *__restrict__ kc, TileMap map, int m0, int d) {
@@ -528,7 +554,7 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves: rows wm * 128, columns wn * 64
   const int l16 = lane & 15, lk = lane >> 4;
   const int G = gridDim.x;
-  const int nk = m / JBK;  // even: m0 % 256 == 0
+  const int nk = (m + JBK - 1) / JBK;
   constexpr int bias = SMALL_K ? 128 : 0;
   // read offsets (bytes) of this lane's row l16, chunk lk (kh = 0) and lk + 4 (kh = 1)
   const int s16 = jswz(l16);
@@ -565,7 +591,10 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
                                                (int)b_off(pc0 + w * 32 + (lane & 31), k), 0, 0, 0);
   };
   auto load = [&](const BigTile &c, int k, int8_t *As, int8_t *Bs) {
-    stage_load(c.ra, c.rb, (int)a_off(c.row0, k), (int)b_off(c.col0, k), m0, As, Bs, wave, lane);
+    if (EDGE)
+      stage_load_edge(c.ra, c.rb, c.row0, c.col0, k, m, As, Bs, wave, lane);
+    else
+      stage_load(c.ra, c.rb, (int)a_off(c.row0, k), (int)b_off(c.col0, k), m0, As, Bs, wave, lane);
   };
   load(t, 0, A0, B0);
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -627,8 +656,9 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       stage_barrier();
     }
-    // nk is even: the last K-step read A1 / B1 and A0 / B0 are free, so the
-    // next tile's first stage goes out before this tile's epilogue
+    // after the K loop's last barrier both stages are free: the next tile's
+    // first stage goes to A0 / B0 before this tile's epilogue, which stages C
+    // in A1 / B1
     const int next = (round + 1) * G + xcd_tile(blockIdx.x, G);
     const bool has_next = next < map.total;
     BigTile tn = t;
@@ -682,7 +712,9 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
       const int ccl = (pass * 32 + (tid >> 4)) & 127;  // pass < 4: A1, else B1
       const unsigned *Cs = reinterpret_cast<const unsigned *>(pass < 4 ? A1 : B1);
       const v4i v = *reinterpret_cast<const v4i *>(Cs + ccl * 64 + ((4 * j16) ^ (4 * (ccl & 15))));
-      *(gv4i *)(Cl + pass * 32 * m0) = v;
+      // EDGE: m % 16 == 0, so a 16-row chunk is wholly inside or outside
+      if (!EDGE || (t.col0 + pass * 32 + (tid >> 4) < m && t.row0 + 16 * j16 < m))
+        *(gv4i *)(Cl + pass * 32 * m0) = v;
     }
     if (!has_next) break;
     t = tn;
@@ -1127,6 +1159,17 @@ hipError_t launch_residue_gemm_bf16(const RGemm *tab, int count, int m, int leve
   return hipGetLastError();
 }
 
+// the large-tile kernel with partial edges for plain leaves (levels = 0) of
+// m % 16 == 0, m <= 16384, when the padding to 256 x 256 tiles adds at most
+// half the area (m >= 512 at worst): at 0.4 of peak it beats the 128 x 128
+// kernel there
+bool big_edge_gemm(int m, int levels, int tile) {
+  if (tile != 0 || levels != 0 || m % 16 != 0 || m % JBM == 0 || m > 16384 || m < 512)
+    return false;
+  const long long padded = (long long)((m + JBM - 1) / JBM) * JBM;
+  return 2 * padded * padded <= 3LL * m * m;
+}
+
 hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, int nmod,
                                const OzakiConsts *kc, hipStream_t s, int tile) {
   if (count <= 0 || m <= 0) return hipSuccess;
@@ -1141,11 +1184,20 @@ hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, i
     if (!ok) return hipErrorInvalidValue;
     const int G = persistent_grid(1, device_cu_count(), map.total);
     if (m <= 16384)
-      hipLaunchKernelGGL((oz_igemm_big_kernel<true>), dim3(G), dim3(JT), 0, s, tab, m, nmod, kc,
-                         map, m0, levels);
+      hipLaunchKernelGGL((oz_igemm_big_kernel<true, false>), dim3(G), dim3(JT), 0, s, tab, m,
+                         nmod, kc, map, m0, levels);
     else
-      hipLaunchKernelGGL((oz_igemm_big_kernel<false>), dim3(G), dim3(JT), 0, s, tab, m, nmod, kc,
-                         map, m0, levels);
+      hipLaunchKernelGGL((oz_igemm_big_kernel<false, false>), dim3(G), dim3(JT), 0, s, tab, m,
+                         nmod, kc, map, m0, levels);
+    return hipGetLastError();
+  }
+  if (big_edge_gemm(m, levels, tile)) {
+    // plain operands of any m % 16 == 0 on the large tiles, partial edges
+    TileMap map = make_tilemap(m, JBM, (long long)count * nmod, ok);
+    if (!ok) return hipErrorInvalidValue;
+    const int G = persistent_grid(1, device_cu_count(), map.total);
+    hipLaunchKernelGGL((oz_igemm_big_kernel<true, true>), dim3(G), dim3(JT), 0, s, tab, m, nmod,
+                       kc, map, m0, levels);
     return hipGetLastError();
   }
   TileMap map = make_tilemap(m, IBM, (long long)count * nmod, ok);

@@ -85,6 +85,7 @@ def test_same_buffers_on_two_streams():
     (caps_amd.MODE_FP64, 2048, 3, "bbb", 1),    # quadtree operands under the cutoff
     (caps_amd.MODE_CRT_INT8, 2048, 2, "bb", 0),  # leaves of 512: 256 vs 128 tiles
     (caps_amd.MODE_CRT_INT8, 4096, 3, "bbb", 1),
+    (caps_amd.MODE_CRT_INT8, 2560, 2, "bd", 0),  # leaves of 640: partial 256-tile edges
 ])
 def test_small_tile_gives_the_same_bits(mode, n, r, pat, cutoff):
     """The autotuner's tile dimension changes no arithmetic: every output

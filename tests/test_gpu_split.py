@@ -191,7 +191,10 @@ def test_crt_bf16_bit_exact_vs_model(n, r, patterns):
 
 
 @pytest.mark.parametrize("n,r,pattern,cutoff", [(4096, 1, "b", 0), (2048, 2, "bb", 1),
-                                                (1536, 2, "bd", 0)])
+                                                (1536, 2, "bd", 0),
+                                                # leaves of 640 and 1344: the INT8 side takes
+                                                # the 256-tile kernel with partial edges
+                                                (2560, 2, "bb", 0), (2688, 1, "b", 0)])
 def test_crt_bf16_equals_int8_mode(n, r, pattern, cutoff):
     """Large leaves (K up to 2048: several FP32 folds) and the quadtree
     operands of a cutoff: the BF16 and INT8 leaf GEMMs give the same bits."""
