@@ -123,13 +123,15 @@ __global__ __launch_bounds__(T) void crt_finalize_kernel(const CrtMats *__restri
 }
 
 // ------------------------------------------------------------------ split
-// A leaf L -> row-major residue leaves; 64 x 64 tiles transposed through LDS.
-// FAST: m % 64 == 0 and the leaf is column-major (bs == 1 or bs == m).
-// LDS: column c of the tile in Ts[c], row r at slot r ^ tswz(c).  The
-// transposed read has lanes on 16 rows and 4 column groups (c >> 4); the
-// swizzle puts the four groups 8 slots apart, so any 16 or 32 consecutive
-// lanes read distinct 8-byte slots of a 512-byte span, and the column-wise
-// write is a permutation of one 512-byte column.
+// A leaf L -> row-major residue leaves, transposed through LDS.
+// FAST (m % 64 == 0 and the leaf column-major, bs == 1 or bs == m): tiles of
+// 32 rows x 128 k; the column-major side moves as 256-byte column pieces, the
+// row-major side as 128-byte row pieces (8 lanes x 16 residues), K chunks past
+// m (m % 128 == 64) skipped.  Column c of the tile in T2[c], row r at slot r ^
+// tswz(c); the transposed read has lanes on 8 rows and 8 column groups, the
+// swizzle puts 4 of the groups 8 slots apart (the other 4 pairs share a
+// slot: a 64-lane 8-byte read spans at least two 256-byte passes anyway).
+// Otherwise 64 x 64 tiles with checked, element-wise accesses.
 __device__ __forceinline__ int tswz(int c) { return ((c >> 4) & 3) << 3; }
 
 template <bool FAST>
@@ -141,48 +143,65 @@ __device__ __forceinline__ void split_a_body(const CrtMats *__restrict__ M,
   int bi, bj;
   leaf_coords(L, M->nrec, bi, bj);
   const long long lbase = (long long)L * m * m;
+  const long long plane = (long long)M->n * M->n;
+  if (FAST) {
+    double(*T2)[32] = reinterpret_cast<double(*)[32]>(&Ts[0][0]);  // [128 k][32 rows]
+    const int tk = (m + 127) / 128;
+    const int r0 = (bx / tk) * 32, k0 = (bx % tk) * 128;
+    for (int idx = threadIdx.x; idx < 32 * 128; idx += T) {
+      const int r = idx & 31, c = idx >> 5;
+      const int lj = k0 + c;
+      T2[c][r ^ tswz(c)] = lj < m ? M->A[lbase + r0 + r + (long long)lj * m] : 0.0;
+    }
+    __syncthreads();
+    const int r = threadIdx.x >> 3, kq = (threadIdx.x & 7) * 16;
+    if (k0 + kq >= m) return;  // m % 16 == 0: the chunk is wholly past the leaf
+    const int li = r0 + r;
+    const int sa = M->sa[bi * m + li];
+    double x[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) x[c] = trunc(ldexp(T2[kq + c][r ^ tswz(kq)], sa));
+    int8_t *dst = M->ra + lbase + (long long)li * m + k0 + kq;
+    for (int l = 0; l < nmod; l++) {
+      const int mi = kc->m[l];
+      const double inv = kc->inv_m[l];
+      const int c22 = kc->c22[l];
+      unsigned w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        w[q] = pack4_lo(sym_residue(x[4 * q], mi, inv, c22),
+                        sym_residue(x[4 * q + 1], mi, inv, c22),
+                        sym_residue(x[4 * q + 2], mi, inv, c22),
+                        sym_residue(x[4 * q + 3], mi, inv, c22));
+      *reinterpret_cast<uint4 *>(dst + l * plane) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    return;
+  }
   const int tiles = (m + 63) / 64;
   const int r0 = (bx % tiles) * 64, k0 = (bx / tiles) * 64;
   for (int idx = threadIdx.x; idx < 64 * 64; idx += T) {
     const int r = idx & 63, c = idx >> 6;
     const int li = r0 + r, lj = k0 + c;
     double v = 0.0;
-    if (FAST)
-      v = M->A[lbase + li + (long long)lj * m];
-    else if (li < m && lj < m)
-      v = M->A[lbase + leaf_elem_offset(m, bs, li, lj)];
+    if (li < m && lj < m) v = M->A[lbase + leaf_elem_offset(m, bs, li, lj)];
     Ts[c][r ^ tswz(c)] = v;
   }
   __syncthreads();
   const int r = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
   const int li = r0 + r;
-  if (!FAST && li >= m) return;
+  if (li >= m) return;
   const int sa = M->sa[bi * m + li];
   double x[16];
 #pragma unroll
   for (int c = 0; c < 16; c++) x[c] = trunc(ldexp(Ts[kq + c][r ^ tswz(kq)], sa));
-  const long long plane = (long long)M->n * M->n;
   int8_t *dst = M->ra + lbase + (long long)li * m + k0 + kq;
   for (int l = 0; l < nmod; l++) {
     const int mi = kc->m[l];
     const double inv = kc->inv_m[l];
     const int c22 = kc->c22[l];
-    if (FAST) {
-      unsigned w[4];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        unsigned v = 0;
-#pragma unroll
-        for (int b = 0; b < 4; b++)
-          v |= (unsigned)(sym_residue(x[4 * q + b], mi, inv, c22) & 0xff) << (8 * b);
-        w[q] = v;
-      }
-      *reinterpret_cast<uint4 *>(dst + l * plane) = make_uint4(w[0], w[1], w[2], w[3]);
-    } else {
-#pragma unroll
-      for (int c = 0; c < 16; c++)
-        if (k0 + kq + c < m) dst[l * plane + c] = (int8_t)sym_residue(x[c], mi, inv, c22);
-    }
+    for (int c = 0; c < 16; c++)
+      if (k0 + kq + c < m) dst[l * plane + c] = (int8_t)sym_residue(x[c], mi, inv, c22);
   }
 }
 
@@ -489,10 +508,12 @@ bool crt_grid(int k, const CrtMats &h, int &gx, int &gy) {
       gx = (h.n + T - 1) / T;
       return false;
     case CRT_SPLIT_A: {
+      const bool fast = plain && h.m % 64 == 0;
       const int tiles = (h.m + 63) / 64;
-      gx = tiles * tiles;
+      // fast: 32-row x 128-k tiles; else 64 x 64
+      gx = fast ? (h.m / 32) * ((h.m + 127) / 128) : tiles * tiles;
       gy = leaves;
-      return plain && h.m % 64 == 0;
+      return fast;
     }
     case CRT_SPLIT_B: {
       const bool fb = plain && h.m % 16 == 0;
