@@ -403,7 +403,8 @@ __global__ __launch_bounds__(T256) void oz_igemm_multi_kernel(const char *__rest
 // directly (MUBUF buffer_load ... lds: each lane's 16 bytes land at the
 // wave's LDS base + 16 * lane), double-buffered, so no registers hold the
 // next stage.  Per MAC this reads 3/4 of the LDS bytes and half the L2 bytes
-// of the 128 x 128 kernel above.  Needs m0 % 256 == 0.
+// of the 128 x 128 kernel above.  Leaves of m0 % 256 == 0, or (EDGE) plain
+// leaves of m % 16 == 0 with partial last tiles.
 //
 // LDS stage layout per operand: 256 rows x 8 chunks of 16 bytes (k), no
 // padding; chunk c of row r sits in slot c ^ swz(r), swz(r) = ((r >> 2) & 3)
@@ -1021,8 +1022,9 @@ oz_bgemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
       if (kt + 1 < nk) lstore(ga, gb, odd ? A0 : A1, odd ? B0 : B1);
       __syncthreads();
     }
-    // the next tile's first stage goes to A0 / B0 (free: nk is even, the last
-    // step read A1 / B1) before this tile's epilogue
+    // after the K loop's last barrier both stages are free: the next tile's
+    // first stage goes to registers now and to A0 / B0 after this tile's C is
+    // staged in A1 / B1
     const int next = (round + 1) * G + xcd_tile(blockIdx.x, G);
     const bool has_next = next < map.total;
     BigTile tn = t;
