@@ -173,11 +173,14 @@ struct Builder {
       // residue leaves are already row-major (A) / column-major (B, C)
       ensure_consts();
       std::vector<RGemm> g(batch.size());
-      for (size_t p = 0; p < batch.size(); p++)
+      const bool segmented = residue_gemm_segmented(m, cutoff, tile, mode == CAPS_MODE_CRT_BF16);
+      for (size_t p = 0; p < batch.size(); p++) {
         g[p] = {reinterpret_cast<const int8_t *>(batch[p].A.p),
                 reinterpret_cast<const int8_t *>(batch[p].B.p),
                 reinterpret_cast<int8_t *>(batch[p].C.p), batch[p].A.ps, batch[p].B.ps,
-                batch[p].C.ps};
+                batch[p].C.ps, batch[p].cseg};
+        if (batch[p].cseg != 128 && !segmented) broken = true;
+      }
       // bs field: 1 = BF16 MFMA leaf products
       push(STEP_OZ_GEMM, moduli, (int)g.size(), m, mode == CAPS_MODE_CRT_BF16 ? 1 : 0,
            add_table(g.data(), g.size() * sizeof(RGemm)), 0, cutoff);
@@ -255,7 +258,7 @@ struct Builder {
       L.ra = reinterpret_cast<int8_t *>(base + RA + plane * nmod * p);
       L.rb = reinterpret_cast<int8_t *>(base + RB + plane * nmod * p);
       L.rc = reinterpret_cast<int8_t *>(base + RC + plane * nmod * p);
-      g[p] = {L.ra, L.rb, L.rc, (long long)plane, (long long)plane, (long long)plane};
+      g[p] = {L.ra, L.rb, L.rc, (long long)plane, (long long)plane, (long long)plane, 128};
     }
     const size_t tab = add_table(t.data(), t.size() * sizeof(OzLeaf));
     const size_t gtab = add_table(g.data(), g.size() * sizeof(RGemm));
@@ -380,6 +383,13 @@ struct Builder {
     const int quadB[7] = {0, 1, -1, -1, -1, 3, -1};  // S1 = B11, S2 = B21, S6 = B22
     const bool fused_b = crt && crt->fused_b;
     Mat LA = alloc_mats(40, q), LB = alloc_mats(fused_b ? 49 : 40, q), P2 = alloc_mats(49, q);
+    // residue products written by the leaf GEMM itself (no level between) on
+    // its 256-tile kernel: the 49 products of a modulus interleave in 128-byte
+    // segments, so that the post-combination reads one contiguous 49 x 128
+    // bytes per modulus for 128 positions
+    const bool seg = residue && nrec - 2 == cutoff && q % 128 == 0 &&
+                     residue_gemm_segmented(sub, cutoff, tile, mode == CAPS_MODE_CRT_BF16);
+    const long long pstep = seg ? 128 : q * es, cseg = seg ? 49 * 128 : 128;
     // the kernels address the 40 written operands of a side as out + k * step
     // (k counts the non-plain (i, j) in order) and product (i, j) as
     // q + (7 i + j) * step
@@ -411,7 +421,7 @@ struct Builder {
         } else {
           b = at(LB, (long long)kb++ * q);
         }
-        next[k] = {a, b, at(P2, k * q)};
+        next[k] = {a, b, {P2.p + k * pstep, P2.ps}, cseg};
       }
     if (ka != 40 || kb != (fused_b ? 0 : 40)) broken = true;
     if (residue) ensure_consts();
@@ -430,10 +440,11 @@ struct Builder {
     rec(next, sub, nrec - 2, depth + 2);
     BBPost post{};
     post.q = P2.p;
-    post.step = q * es;
+    post.step = pstep;
     post.sq = P2.ps;
+    post.cseg = cseg;
     for (int k = 0; k < 49; k++)
-      if (next[k].C.p != P2.p + k * q * es || next[k].C.ps != P2.ps) broken = true;
+      if (next[k].C.p != P2.p + k * pstep || next[k].C.ps != P2.ps) broken = true;
     if (!residue) {
       post.c = reinterpret_cast<double *>(pb.C.p);
     } else {

@@ -25,9 +25,11 @@ struct Mat {
   long long ps;
 };
 
-// one sub-problem of a batched level: C = A * B, all s x s in CAPS layout
+// one sub-problem of a batched level: C = A * B, all s x s in CAPS layout.
+// cseg: residue leaves only, C's segment stride (RGemm::cseg; 128 = plain)
 struct Prob {
   Mat A, B, C;
+  long long cseg = 128;
 };
 
 enum StepKind : int {

@@ -31,6 +31,10 @@ struct BBPost {
   const void *q;          // the 49 depth-2 products (i, j), product 7 i + j at (7 i + j) * step
   long long step;         // bytes
   long long sq;           // residue mode: plane stride
+  // residue mode: position u of a product at (u >> 7) * cseg + (u & 127)
+  // (128: plain planes; 49 * 128 with step = 128: the products interleaved
+  // in 128-byte segments, as the 256-tile leaf GEMM writes them)
+  long long cseg;
   double *c;              // FP64 mode: C's 16 depth-2 sub-blocks at (4a + b) * q
   // residue mode: reconstruct into FP64 C (CAPS layout) directly
   double *out_c;

@@ -284,7 +284,9 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
     const int mi = kc->m[l], c22 = kc->c22[l];
     // this lane's word of product 0; product k lies k * step bytes further
     // (a wave-uniform offset)
-    const char *qb = reinterpret_cast<const char *>(P->q) + l * P->sq + 4 * u4;
+    const long long u = 4 * u4;
+    const char *qb =
+        reinterpret_cast<const char *>(P->q) + l * P->sq + (u >> 7) * P->cseg + (u & 127);
     const long long step = P->step;
     unsigned w[49];
 #pragma unroll

@@ -63,12 +63,16 @@ struct OzLeaf {
 
 // One leaf product of the INT8 GEMM step: nmod residue planes per operand,
 // plane l of an operand at base + l * stride (bytes).  a: row-major m x m
-// (k contiguous), b and c: column-major m x m.
+// (k contiguous), b and c: column-major m x m.  Element e of C's plane l sits
+// at c + l * sc + (e >> 7) * cseg + (e & 127): cseg = 128 is the plain plane;
+// a larger cseg interleaves the 128-byte segments of several products (the
+// 256-tile kernels only: residue_gemm_segmented says which launches take them).
 struct RGemm {
   const int8_t *a;
   const int8_t *b;
   int8_t *c;
   long long sa, sb, sc;
+  long long cseg;
 };
 
 // Host: moduli and CRT constants for nmod moduli (2..OZ_MAX_MODULI).
@@ -100,6 +104,9 @@ int residue_gemm_check(int m, int levels);
 // launch_residue_gemm takes the large tiles with partial edges for this
 // (plain leaves, m % 16 == 0, little padding)
 bool big_edge_gemm(int m, int levels, int tile);
+// launch_residue_gemm (INT8) / launch_residue_gemm_bf16 (bf16) runs these
+// leaves on the 256-tile kernel for whole tiles, which honours RGemm::cseg
+bool residue_gemm_segmented(int m, int levels, int tile, bool bf16);
 hipError_t launch_residue_gemm_multi(const char *d_tables, const VSeg *d_segs, int nseg,
                                      int total_tiles, int nmod, const OzakiConsts *d_consts,
                                      hipStream_t s);

@@ -499,6 +499,7 @@ struct BigTile {
   __amdgpu_buffer_rsrc_t ra, rb;  // buffer descriptors of the operand planes
   int mi, c14, c22, row0, col0, ti, tj;
   float mf, invf;
+  long long cseg;  // C segment stride (RGemm::cseg)
 };
 
 // a uniform pointer kept in scalar registers (buffer descriptors built from a
@@ -522,6 +523,7 @@ __device__ __forceinline__ BigTile big_tile(const RGemm *__restrict__ tab, const
   t.Ap = uniform_ptr(tab[leaf].a + l * tab[leaf].sa);  // row-major leaves
   t.Bp = uniform_ptr(tab[leaf].b + l * tab[leaf].sb);  // column-major leaves
   t.Cp = tab[leaf].c + l * tab[leaf].sc;
+  t.cseg = tab[leaf].cseg;
   t.mi = kc->m[l];
   t.mf = (float)t.mi;
   t.invf = 1.0f / t.mf;
@@ -705,9 +707,14 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     stage_barrier();
     // lane: 16 bytes (chunk j16) of tile column pass * 32 + (tid >> 4)
+    // element e of the C plane at (e >> 7) cseg + (e & 127); a 16-byte chunk
+    // (e % 16 == 0) stays in one segment, and a pass moves 32 m0 elements, a
+    // multiple of 128 (m0 % 16 == 0)
     const int j16 = tid & 15;
-    int8_t *Cl = t.Cp + quad_leaf(t.row0 / m0, t.col0 / m0, d) * mm0 +
-                 (long long)(t.col0 % m0 + (tid >> 4)) * m0 + (t.row0 % m0) + 16 * j16;
+    const long long e0 = quad_leaf(t.row0 / m0, t.col0 / m0, d) * mm0 +
+                         (long long)(t.col0 % m0 + (tid >> 4)) * m0 + (t.row0 % m0) + 16 * j16;
+    int8_t *Cl = t.Cp + (e0 >> 7) * t.cseg + (e0 & 127);
+    const long long pstep = (long long)(m0 >> 2) * t.cseg;
 #pragma unroll
     for (int pass = 0; pass < 8; pass++) {
       const int ccl = (pass * 32 + (tid >> 4)) & 127;  // pass < 4: A1, else B1
@@ -715,7 +722,7 @@ oz_igemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
       const v4i v = *reinterpret_cast<const v4i *>(Cs + ccl * 64 + ((4 * j16) ^ (4 * (ccl & 15))));
       // EDGE: m % 16 == 0, so a 16-row chunk is wholly inside or outside
       if (!EDGE || (t.col0 + pass * 32 + (tid >> 4) < m && t.row0 + 16 * j16 < m))
-        *(gv4i *)(Cl + pass * 32 * m0) = v;
+        *(gv4i *)(Cl + pass * pstep) = v;
     }
     if (!has_next) break;
     t = tn;
@@ -1049,15 +1056,18 @@ oz_bgemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
       }
     if (has_next) lstore(ga, gb, A0, B0);
     __syncthreads();
+    // C segments as in the INT8 kernel
     const int j16 = tid & 15;
-    int8_t *Cl = t.Cp + quad_leaf(t.row0 / m0, t.col0 / m0, d) * mm0 +
-                 (long long)(t.col0 % m0 + (tid >> 4)) * m0 + (t.row0 % m0) + 16 * j16;
+    const long long e0 = quad_leaf(t.row0 / m0, t.col0 / m0, d) * mm0 +
+                         (long long)(t.col0 % m0 + (tid >> 4)) * m0 + (t.row0 % m0) + 16 * j16;
+    int8_t *Cl = t.Cp + (e0 >> 7) * t.cseg + (e0 & 127);
+    const long long pstep = (long long)(m0 >> 2) * t.cseg;
 #pragma unroll
     for (int pass = 0; pass < 8; pass++) {
       const int ccl = (pass * 32 + (tid >> 4)) & 127;  // pass < 4: A1, else B1
       const unsigned *Cs = reinterpret_cast<const unsigned *>(pass < 4 ? A1 : B1);
       const v4i v = *reinterpret_cast<const v4i *>(Cs + ccl * 64 + ((4 * j16) ^ (4 * (ccl & 15))));
-      *(gv4i *)(Cl + pass * 32 * m0) = v;
+      *(gv4i *)(Cl + pass * pstep) = v;
     }
     if (!has_next) break;
     t = tn;
@@ -1172,6 +1182,18 @@ bool big_edge_gemm(int m, int levels, int tile) {
   return 2 * padded * padded <= 3LL * m * m;
 }
 
+// whole 256 x 256 tiles: one block (8 waves, 128 KB of LDS) per CU; buffer
+// offsets are 32-bit, so an operand quadtree stays below 2 GiB
+static bool big_int8_gemm(int m, int levels, int tile) {
+  const int m0 = m >> levels;
+  return tile == 0 && m0 % JBM == 0 && (long long)m * m < (1LL << 31);
+}
+
+bool residue_gemm_segmented(int m, int levels, int tile, bool bf16) {
+  if (m <= 0 || levels < 0 || ((m >> levels) << levels) != m) return false;
+  return bf16 ? (m >> levels) % JBM == 0 : big_int8_gemm(m, levels, tile);
+}
+
 hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, int nmod,
                                const OzakiConsts *kc, hipStream_t s, int tile) {
   if (count <= 0 || m <= 0) return hipSuccess;
@@ -1179,9 +1201,7 @@ hipError_t launch_residue_gemm(const RGemm *tab, int count, int m, int levels, i
   const int m0 = m >> levels;
   if (levels < 0 || (m0 << levels) != m) return hipErrorInvalidValue;
   bool ok;
-  if (tile == 0 && m0 % JBM == 0 && (long long)m * m < (1LL << 31)) {
-    // large tiles: one block (8 waves, 128 KB of LDS) per CU; buffer offsets
-    // are 32-bit, so an operand quadtree stays below 2 GiB
+  if (big_int8_gemm(m, levels, tile)) {
     TileMap map = make_tilemap(m, JBM, (long long)count * nmod, ok);
     if (!ok) return hipErrorInvalidValue;
     const int G = persistent_grid(1, device_cu_count(), map.total);
