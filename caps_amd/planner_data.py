@@ -7,10 +7,13 @@ product) are recorded.
     python -m caps_amd.planner_data --split train --shapes 170 --part 0 --parts 4 \\
         --out gpurun_out/planner_train_0.npz
 
-Shapes: n = 256 round(exp(U[ln 512, ln 16384]) / 256); layout depth r = the
-largest with leaves n / 2^r >= 512 and n divisible by 2^r (at least 0); entries u 2^e with
-u uniform [-1, 1), e uniform integer in [-spread, spread], spread uniform in
-{0, ..., 20}.  Training and held-out shapes come from different seeds.
+Shapes (splits train / test): n = 256 round(exp(U[ln 512, ln 16384]) / 256);
+layout depth r = the largest with leaves n / 2^r >= 512 and n divisible by 2^r
+(at least 0).  Splits batch_train / batch_test cover the sizes of uneven
+batches: n = 64 round(exp(U[ln 128, ln 4096]) / 64), r uniform over the depths
+with n divisible by 2^r and leaves of at least 64.  Entries u 2^e with u
+uniform [-1, 1), e uniform integer in [-spread, spread], spread uniform in
+{0, ..., 20}.  Every split has its own seed.
 Candidates: plain FP64 and the residue-domain mode (moduli 12..16, or the
 subset given by --crt-moduli), every pattern b^k d^(L-k) and every
 admissible cutoff; a candidate whose plan does not fit the device's free
@@ -29,16 +32,22 @@ import numpy as np
 
 import caps_amd
 
-SEEDS = {"train": 1000, "test": 2000}
+SEEDS = {"train": 1000, "test": 2000, "batch_train": 3000, "batch_test": 4000}
 
 
 def shape(split: str, i: int):
     rng = np.random.default_rng(SEEDS[split] * 100003 + i)
-    n = int(256 * np.round(np.exp(rng.uniform(np.log(512), np.log(16384))) / 256))
-    n = min(max(n, 512), 16384)
-    r = 0
-    while n % (2 << r) == 0 and (n >> (r + 1)) >= 512:
-        r += 1
+    if split.startswith("batch"):
+        n = int(64 * np.round(np.exp(rng.uniform(np.log(128), np.log(4096))) / 64))
+        n = min(max(n, 128), 4096)
+        ds = [r for r in range(12) if n % (1 << r) == 0 and (n >> r) >= 64]
+        r = int(ds[int(rng.integers(0, len(ds)))])
+    else:
+        n = int(256 * np.round(np.exp(rng.uniform(np.log(512), np.log(16384))) / 256))
+        n = min(max(n, 512), 16384)
+        r = 0
+        while n % (2 << r) == 0 and (n >> (r + 1)) >= 512:
+            r += 1
     spread = int(rng.integers(0, 21))
     return n, r, spread
 

@@ -4,6 +4,9 @@ held-out shapes against the exhaustive autotuner.
     python -m caps_amd.planner_train --train RUNS_TRAIN.npz ... --test RUNS_TEST.npz ... \\
         --epochs 300 --model caps_amd/planner_model.npz
 
+Each --test file is scored on its own, also with the choice restricted to
+plain FP64 candidates under --fp64-target (the FP64 mode of uneven batches).
+
 Runs come from caps_amd.planner_data.  The model is the library's 3-layer
 perceptron (8 -> 64 -> 64 -> 2, ReLU, FP32), trained by the library's own HIP
 forward / backward / Adam kernels (caps_mlp_train) on normalised features
@@ -27,6 +30,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 
 import numpy as np
@@ -122,7 +126,10 @@ def _reference_index(rows_idx, runs):
     return None
 
 
-def evaluate(model, norm, test) -> dict:
+def evaluate(model, norm, test, target: float = TARGET, modes=None) -> dict:
+    """Scores per held-out shape (see the module docstring); `modes`
+    restricts every choice to candidates of those modes (e.g. FP64 only,
+    with its own error target)."""
     P = predict(model, norm, test["features"])
     T = targets(test)
     res = {"rmse_log2_ms": float(np.sqrt(np.mean((P[:, 0] - T[:, 0]) ** 2))),
@@ -134,13 +141,17 @@ def evaluate(model, norm, test) -> dict:
     no_feasible = 0
     for s in shapes:
         idx = np.nonzero(test["shape"] == s)[0]
+        if modes is not None:
+            idx = idx[np.isin(test["mode"][idx], list(modes))]
+            if len(idx) == 0:
+                continue
         ms, err = test["ms"][idx], test["err"][idx]
-        feas = err <= TARGET
+        feas = err <= target
         if not feas.any():
             no_feasible += 1
             continue
         best = ms[feas].min()
-        ok = P[idx, 1] <= np.log10(TARGET) - MARGIN
+        ok = P[idx, 1] <= np.log10(target) - MARGIN
         if ok.any():
             k = idx[ok][np.argmin(P[idx[ok], 0])]
         else:
@@ -151,12 +162,16 @@ def evaluate(model, norm, test) -> dict:
             if j is None:
                 continue
             reg[name].append(test["ms"][j] / best - 1.0)
-            viol[name] += int(test["err"][j] > TARGET)
+            viol[name] += int(test["err"][j] > target)
     for name, r in reg.items():
-        r = np.array(r) if r else np.array([np.nan])
-        res[name] = {"shapes": int(np.isfinite(r).sum()), "mean_regret": float(np.nanmean(r)),
-                     "median_regret": float(np.nanmedian(r)),
-                     "p90_regret": float(np.nanpercentile(r, 90)), "violations": viol[name]}
+        if not r:  # no row of this rule among the scored modes
+            res[name] = {"shapes": 0, "mean_regret": float("nan"), "median_regret": float("nan"),
+                         "p90_regret": float("nan"), "violations": 0}
+            continue
+        r = np.array(r)
+        res[name] = {"shapes": int(len(r)), "mean_regret": float(np.mean(r)),
+                     "median_regret": float(np.median(r)),
+                     "p90_regret": float(np.percentile(r, 90)), "violations": viol[name]}
     res["shapes"] = int(len(shapes))
     res["model_fallbacks_to_exhaustive"] = fallbacks
     res["shapes_without_feasible_candidate"] = no_feasible
@@ -186,14 +201,20 @@ def main(argv=None) -> int:
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--model", default=None, help="save the trained model here (.npz)")
+    ap.add_argument("--fp64-target", type=float, default=1e-12,
+                    help="also score FP64-only choices against this error target")
     a = ap.parse_args(argv)
-    train, test = load_runs(a.train), load_runs(a.test)
+    train = load_runs(a.train)
     model, norm, losses = fit(train, a.epochs, a.seed)
-    rep = evaluate(model, norm, test)
-    rep["train_rows"] = int(len(train["ms"]))
-    rep["test_rows"] = int(len(test["ms"]))
-    rep["train_shapes"] = int(len(np.unique(train["shape"])))
-    rep["final_losses"] = losses
+    # each held-out file on its own (shape ids are unique within a split only)
+    rep = {"train_rows": int(len(train["ms"])), "final_losses": losses}
+    for path in a.test:
+        test = load_runs([path])
+        name = os.path.basename(path)
+        rep[name] = evaluate(model, norm, test)
+        rep[name]["test_rows"] = int(len(test["ms"]))
+        rep[name]["fp64_only"] = evaluate(model, norm, test, a.fp64_target,
+                                          modes=(caps_amd.MODE_FP64,))
     if a.model:
         save_model(a.model, model, norm)
     print(json.dumps(rep, indent=1))

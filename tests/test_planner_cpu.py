@@ -116,3 +116,39 @@ def test_planner_shapes():
     test = [pd.shape("test", i) for i in range(50)]
     assert train != test
     assert pd.shape("train", 7) == pd.shape("train", 7)
+
+
+def test_batch_planner_shapes():
+    """The batch-domain shapes: n a multiple of 64 in [128, 4096], a depth
+    drawn from those with leaves >= 64 dividing n, spread in 0..20; the two
+    batch splits differ from each other."""
+    from caps_amd import planner_data as pd
+    seen = set()
+    for split in ("batch_train", "batch_test"):
+        for i in range(300):
+            n, r, spread = pd.shape(split, i)
+            assert n % 64 == 0 and 128 <= n <= 4096
+            assert n % (1 << r) == 0 and (n >> r) >= 64
+            assert 0 <= spread <= 20
+            seen.add((n, r))
+    # depth 6 needs n = 4096, the top of the range; 0..5 all turn up
+    assert {0, 1, 2, 3, 4, 5} <= {r for _, r in seen}
+    assert [pd.shape("batch_train", i) for i in range(50)] != \
+        [pd.shape("batch_test", i) for i in range(50)]
+
+
+def test_mode_restricted_evaluation():
+    """evaluate(modes=FP64, target) chooses among FP64 rows only: with a looser
+    target the fast FP64 candidate is feasible and the perfect predictor
+    takes it; the no-model default (a residue-mode row) is not scored."""
+    runs = _runs()
+    truth = {0: (np.log2(10.0), np.log10(2e-14)), 1: (np.log2(8.0), np.log10(3e-14)),
+             2: (np.log2(6.0), np.log10(5e-15)), 3: (np.log2(4.0), np.log10(4e-15))}
+    rep = pt.evaluate(_Stand(truth), NORM, runs, target=1e-12, modes=(caps_amd.MODE_FP64,))
+    assert rep["model"]["mean_regret"] == 0.0 and rep["model"]["violations"] == 0
+    assert rep["default"]["shapes"] == 0
+    assert abs(rep["reference"]["mean_regret"] - 0.25) < 1e-12
+    # under 1e-14 no FP64 row is feasible: every shape is skipped
+    rep = pt.evaluate(_Stand(truth), NORM, runs, modes=(caps_amd.MODE_FP64,))
+    assert rep["shapes_without_feasible_candidate"] == 3
+
