@@ -55,6 +55,14 @@ __global__ __launch_bounds__(LT) void layout_kernel(const double *__restrict__ s
   }
 }
 
+// LDS pitch (doubles) of a tile column.  The CAPS side walks a block column
+// as runs of BS rows in BS consecutive tile columns; a 64-bit LDS read serves
+// 16 lanes at a time, here 16 / BS columns of BS doubles each, and with the
+// pitch 64 + BS those columns start 2 BS banks apart: the 16 lanes read 32
+// consecutive banks (pitch 65 put the columns 2 banks apart and overlapped
+// them).  Runs of 64 (BS = 0) read one column.
+constexpr int tile_pitch(int BS) { return TS + (BS > 0 ? BS : 1); }
+
 // Tile (ti, tj) of one n x n matrix: cm is column-major (leading dimension
 // n), caps the CAPS layout of depth nrec with blocks of bs (BS > 0: bs = BS
 // divides 64; BS == 0: bs is a multiple of 64).  The leaf size m = n >> nrec
@@ -62,7 +70,8 @@ __global__ __launch_bounds__(LT) void layout_kernel(const double *__restrict__ s
 template <bool TO_CAPS, int BS>
 __device__ __forceinline__ void layout_tile(const double *__restrict__ src,
                                             double *__restrict__ dst, int n, int nrec, int bs,
-                                            int ti, int tj, double (&S)[TS][TS + 1]) {
+                                            int ti, int tj,
+                                            double (&S)[TS][tile_pitch(BS)]) {
   const int m = n >> nrec;
   const int i0 = ti * TS, j0 = tj * TS;  // tile origin in the matrix
   const int li0 = i0 % m, lj0 = j0 % m;  // and in its leaf
@@ -120,7 +129,7 @@ __global__ __launch_bounds__(LT) void layout_tile_kernel(const ReorderPtrs *__re
                                                          const double *__restrict__ src1,
                                                          double *__restrict__ dst1, int n,
                                                          int nrec, int bs) {
-  __shared__ double S[TS][TS + 1];
+  __shared__ double S[TS][tile_pitch(BS)];
   const double *src = tab ? tab[blockIdx.y].src : src1;
   double *dst = tab ? tab[blockIdx.y].dst : dst1;
   const int t1 = n / TS, tiles = t1 * t1;
