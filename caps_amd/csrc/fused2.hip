@@ -264,9 +264,22 @@ __global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restric
 // phases through LDS (each product plane is read in PPOS-byte segments:
 // 128 keeps them whole cache lines):
 //   1. lane per (4 positions, modulus): 49 product words, both levels of the
-//      post-combination, the 16 residues of C's sub-blocks into LDS;
+//      post-combination (on 16-bit pairs), the 16 residues of C's sub-blocks
+//      into LDS;
 //   2. lane per (position, sub-block of C), consecutive lanes on consecutive
 //      positions: CRT over the moduli from LDS, unscale, FP64 store.
+typedef short v2s __attribute__((ext_vector_type(2)));
+
+// the signed bytes 0, 1 and 2, 3 of w as pairs of 16-bit integers: v_perm_b32
+// selectors 8 and 9 replicate the sign bit of bytes 1 and 3 of its second
+// operand, 10 and 11 those of bytes 1 and 3 of its first (here w << 8, whose
+// bytes 1 and 3 are bytes 0 and 2 of w)
+__device__ __forceinline__ void sext_pairs(unsigned w, v2s &lo, v2s &hi) {
+  const unsigned s = w << 8;
+  lo = __builtin_bit_cast(v2s, __builtin_amdgcn_perm(s, w, 0x08010a00u));
+  hi = __builtin_bit_cast(v2s, __builtin_amdgcn_perm(s, w, 0x09030b02u));
+}
+
 constexpr int PPOS = 128;
 constexpr int PG = PPOS / 4;  // 4-position words per block
 static_assert(T % PPOS == 0, "phase 2 maps a lane to one position");
@@ -291,31 +304,37 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
     unsigned w[49];
 #pragma unroll
     for (int k = 0; k < 49; k++) w[k] = *(gcuint *)(qb + k * step);
-    int res[16][4];  // [sub-block of C][position]
+    // both levels on pairs of 16-bit integers, positions (0, 1) and (2, 3) of
+    // the words (|level 1| <= 512, |level 2| <= 2048)
+    v2s l1[7][4][2];  // [level-1 operand i][sub-position bb][pair]
 #pragma unroll
-    for (int byte = 0; byte < 4; byte++) {
-      int l1[7][4];
+    for (int i = 0; i < 7; i++) {
+      v2s qlo[7], qhi[7], clo[4], chi[4];
 #pragma unroll
-      for (int i = 0; i < 7; i++) {
-        int qv[7], c[4];
-#pragma unroll
-        for (int j = 0; j < 7; j++) qv[j] = (int)__builtin_amdgcn_sbfe(w[7 * i + j], 8 * byte, 8);
-        Ew<EW_BFS_POST>::apply(qv, c);
-#pragma unroll
-        for (int bb = 0; bb < 4; bb++) l1[i][bb] = c[bb];  // |.| <= 512
-      }
+      for (int j = 0; j < 7; j++) sext_pairs(w[7 * i + j], qlo[j], qhi[j]);
+      Ew<EW_BFS_POST>::apply(qlo, clo);
+      Ew<EW_BFS_POST>::apply(qhi, chi);
 #pragma unroll
       for (int bb = 0; bb < 4; bb++) {
-        int qv[7], c[4];
-#pragma unroll
-        for (int i = 0; i < 7; i++) qv[i] = l1[i][bb];
-        Ew<EW_BFS_POST>::apply(qv, c);  // |.| <= 2048
-#pragma unroll
-        for (int aa = 0; aa < 4; aa++) res[4 * aa + bb][byte] = mod_k(c[aa], c22, mi);
+        l1[i][bb][0] = clo[bb];
+        l1[i][bb][1] = chi[bb];
       }
     }
 #pragma unroll
-    for (int o = 0; o < 16; o++) R[l][o][g] = pack4_lo(res[o][0], res[o][1], res[o][2], res[o][3]);
+    for (int bb = 0; bb < 4; bb++) {
+      v2s qlo[7], qhi[7], clo[4], chi[4];
+#pragma unroll
+      for (int i = 0; i < 7; i++) {
+        qlo[i] = l1[i][bb][0];
+        qhi[i] = l1[i][bb][1];
+      }
+      Ew<EW_BFS_POST>::apply(qlo, clo);
+      Ew<EW_BFS_POST>::apply(qhi, chi);
+#pragma unroll
+      for (int aa = 0; aa < 4; aa++)
+        R[l][4 * aa + bb][g] = pack4_lo(mod_k(clo[aa].x, c22, mi), mod_k(clo[aa].y, c22, mi),
+                                        mod_k(chi[aa].x, c22, mi), mod_k(chi[aa].y, c22, mi));
+    }
   }
   __syncthreads();
 

@@ -144,7 +144,9 @@ def test_crt_mode_cutoff_bit_exact_vs_model():
 
 def test_crt_large_tile_gemm_bit_exact_vs_model():
     """Leaves of 256 and quadtree operands of 512 (cutoff) take the 256 x 256
-    tile residue GEMM: same bits as the exact-integer model."""
+    tile residue GEMM (under 'bb' with no cutoff it writes the 49 products of a
+    modulus interleaved in 128-byte segments): same bits as the exact-integer
+    model."""
     n, nrec = 1024, 2
     A, B = realdata(n, 41)
     want = split.crt_mode(A, B, 14)
@@ -192,6 +194,9 @@ def test_crt_bf16_bit_exact_vs_model(n, r, patterns):
 
 @pytest.mark.parametrize("n,r,pattern,cutoff", [(4096, 1, "b", 0), (2048, 2, "bb", 1),
                                                 (1536, 2, "bd", 0),
+                                                # leaves of 256 under 'bb': both 256-tile
+                                                # kernels write the products interleaved
+                                                (1024, 2, "bb", 0),
                                                 # leaves of 640 and 1344: the INT8 side takes
                                                 # the 256-tile kernel with partial edges
                                                 (2560, 2, "bb", 0), (2688, 1, "b", 0)])
