@@ -423,10 +423,11 @@ def main():
                    "plain_leaf_gemm_tflops": round(f64_ach, 2),
                    "plain_leaf_gemm_frac_fp64_peak": round(f64_ach / PEAK_FP64_TFLOPS, 4)},
         "roofline": {"bound": "mfma",
-                     "kernel": "oz_igemm_big_kernel<true> (INT8 MFMA, all leaves x moduli)",
+                     "kernel": "oz_igemm_big_kernel<true, false> (INT8 MFMA, all leaves x "
+                               "moduli)",
                      "achieved": round(achieved, 2), "peak": PEAK_INT8_TOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_INT8_TOPS, 4),
-                     "traffic": profiled_traffic("oz_igemm_big_kernel<true>"),
+                     "traffic": profiled_traffic("oz_igemm_big_kernel<true, false>"),
                      "note": "INT8 multiply-adds counted as 2 ops; per launch "
                              f"{int_ops:.4e} ops = moduli x 2 x 49 x 2048^3, "
                              f"{gemm_ms:.3f} ms"},
