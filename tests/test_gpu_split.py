@@ -218,3 +218,35 @@ def test_crt_bf16_equals_int8_mode(n, r, pattern, cutoff):
         torch.cuda.synchronize()
         outs.append(dC.cpu().numpy())
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("n,nrec,pattern,cutoff", [(2048, 3, "bbb", 1), (4096, 2, "bb", 0)])
+def test_crt_interleaved_products_equal_plain(n, nrec, pattern, cutoff):
+    """Under two leading 'b' levels the 256-tile leaf GEMMs write the 49
+    depth-2 products of a modulus interleaved in 128-byte segments (here on
+    2 x 2 quadtrees of 256-leaves below a cutoff, and on leaves of 1024); the
+    unfused path keeps plain product planes.  INT8 and BF16 leaf products, fused
+    and unfused, give the same bits, within the error bar of the 16-modulus
+    reference product."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(n + 7 * nrec)
+    u = torch.rand(2 * n * n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    e = torch.randint(-10, 11, (2 * n * n,), generator=g, device="cuda", dtype=torch.int32)
+    x = torch.ldexp(u, e.to(torch.float64))
+    dA, dB = x[: n * n].contiguous(), x[n * n:].contiguous()
+    d = caps_amd.MatDescriptor.one_device(n, nrec, 1)
+    outs = []
+    for mode, flags in ((CRT, caps_amd.FLAG_NO_FUSION), (CRT, 0), (BF16, 0)):
+        print(f"[gpu] crt n={n} nrec={nrec} pattern={pattern} cutoff={cutoff} mode={mode} "
+              f"flags={flags}", flush=True)
+        dC = torch.empty_like(dA)
+        caps_amd.multiply_device(dA, dB, dC, d, pattern, mode=mode, moduli=14, flags=flags,
+                                 cutoff=cutoff)
+        torch.cuda.synchronize()
+        outs.append(dC)
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+    ref = torch.empty_like(dA)
+    caps_amd.reference_product(dA, dB, ref, d)
+    torch.cuda.synchronize()
+    assert caps_amd.normwise_error(outs[1], ref, d) <= 1e-14
+
