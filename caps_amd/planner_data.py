@@ -5,7 +5,7 @@ its time (HIP events) and normwise error (against the 16-modulus residue
 product) are recorded.
 
     python -m caps_amd.planner_data --split train --shapes 170 --part 0 --parts 4 \\
-        --out gpurun_out/planner_train_0.npz
+        --out OUT/planner_train_0.npz
 
 Shapes (splits train / test): n = 256 round(exp(U[ln 512, ln 16384]) / 256);
 layout depth r = the largest with leaves n / 2^r >= 512 and n divisible by 2^r

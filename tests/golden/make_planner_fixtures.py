@@ -1,11 +1,12 @@
 """Compact the planner measurements into the fixtures the planner tests read.
 
 The measurements are made on an MI355X by caps_amd.planner_data (one npz
-per part, written under gpurun_out/); this script concatenates the parts of
-a split and keeps the columns with small dtypes:
+per part, written to its --out path, OUT/ below); this script concatenates
+the parts of a split and keeps the columns with small dtypes:
 
     python tests/golden/make_planner_fixtures.py train.npz OUT/planner_train_1.npz
     python tests/golden/make_planner_fixtures.py test.npz OUT/planner_test_0.npz ...
+    python tests/golden/make_planner_fixtures.py planner_runs_batch_train.npz OUT/planner_batch_train.npz
 
 Rows: one per (shape, candidate): shape id, n, layout depth, exponent
 spread, the candidate's feature vector, measured ms, normwise error against
