@@ -261,8 +261,9 @@ __global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restric
 
 // ---------------------------------------------------------------- residue post + CRT
 // One block per PPOS consecutive positions of the depth-2 sub-blocks, two
-// phases through LDS (each product plane is read in PPOS-byte segments:
-// 128 keeps them whole cache lines):
+// phases through LDS (each product is read in PPOS-byte segments: 128 keeps
+// them whole cache lines; with cseg = 49 x 128 the 49 segments of a modulus
+// are one contiguous run):
 //   1. lane per (4 positions, modulus): 49 product words, both levels of the
 //      post-combination (on 16-bit pairs), the 16 residues of C's sub-blocks
 //      into LDS;
