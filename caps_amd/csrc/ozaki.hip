@@ -1022,6 +1022,15 @@ oz_bgemm_big_kernel(const RGemm *__restrict__ tab, int m, int nmod,
 #pragma unroll
         for (int c = 0; c < 4; c++)
           acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[r], b1[c], acc[r][c], 0, 0, 0);
+      // as in the INT8 kernel: the first MFMAs' 6 fragment reads, then one
+      // read per two MFMAs (1.3 % faster than all 24 reads up front)
+      __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+#pragma unroll
+      for (int i = 0; i < 18; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, 28, 0);
       if ((kt + 1) % fold_every == 0) {
 #pragma unroll
         for (int r = 0; r < 8; r++)
