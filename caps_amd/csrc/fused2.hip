@@ -353,18 +353,25 @@ __global__ __launch_bounds__(T) void bb_post_crt_kernel(const BBPost *__restrict
   const long long dst_in_sub = L2 * mm0 + leaf_elem_offset(m0, bs, li, lj);
   const int ri = bi * m0 + li, cj = bj * m0 + lj;
   const unsigned lds_shift = 8 * (pl & 3);
-  for (int o = o0; o < 16; o += T / PPOS) {
+  // outputs o = o0 + 2 i (T / PPOS = 2): sub-block (aa, bb) = (i >> 1, o0 + 2 (i & 1)),
+  // so the lane's 8 outputs use 2 row and 4 column exponents, loaded once
+  static_assert(T / PPOS == 2, "outputs o0 + 2 i");
+  const gcint *ea = (const gcint *)P->sa + o0 * sub + ri;
+  const gcint *eb = (const gcint *)P->sb + cj;
+  const int xa[2] = {ea[0], ea[2 * sub]};
+  const int xb[4] = {eb[0], eb[sub], eb[2 * sub], eb[3 * sub]};
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int o = o0 + 2 * i;
     double S[NP];
 #pragma unroll
     for (int p = 0; p < NP; p++) S[p] = 0.0;
     for (int l = 0; l < nmod; l++)
       crt_acc<NP>(S, (int)__builtin_amdgcn_sbfe(R[l][o][pl >> 2], lds_shift, 8), l, kc);
-    const int aa = o >> 2, bb = o & 3;
-    const int gi = (aa & 1) * 2 * sub + (bb & 1) * sub + ri;
-    const int gj = (aa >> 1) * 2 * sub + (bb >> 1) * sub + cj;
     const double X = crt_finish<NP>(S, kc);
-    __builtin_nontemporal_store(ldexp(X, -(((gcint *)P->sa)[gi] + ((gcint *)P->sb)[gj])),
-                                (gdouble *)P->out_c + o * q + dst_in_sub);
+    // row exponent of sub-block row (aa & 1, bb & 1), column of (aa >> 1, bb >> 1)
+    const int e = xa[(i >> 1) & 1] + xb[2 * (i >> 2) + (i & 1)];
+    __builtin_nontemporal_store(ldexp(X, -e), (gdouble *)P->out_c + o * q + dst_in_sub);
   }
 }
 
