@@ -37,6 +37,7 @@ struct CrtMats {
   double *pb;  // column partials of B: [2^nrec][n] pairs
   int n, nrec, bs, m, nmod, chunks;
   int fused_b;  // B's split is fused with the top pre-additions (no rb planes)
+  int fused_a;  // A's split is fused with the top pre-additions (no ra planes)
 };
 
 // residue Winograd step: like EwPtrs with a plane stride per pointer

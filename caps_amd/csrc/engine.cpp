@@ -381,8 +381,9 @@ struct Builder {
     // plain sub-blocks passed by pointer: level-1 / level-2 operand -> quadrant
     const int quadA[7] = {0, 2, -1, -1, -1, -1, 3};  // T1 = A11, T2 = A12, T7 = A22
     const int quadB[7] = {0, 1, -1, -1, -1, 3, -1};  // S1 = B11, S2 = B21, S6 = B22
-    const bool fused_b = crt && crt->fused_b;
-    Mat LA = alloc_mats(40, q), LB = alloc_mats(fused_b ? 49 : 40, q), P2 = alloc_mats(49, q);
+    const bool fused_b = crt && crt->fused_b, fused_a = crt && crt->fused_a;
+    Mat LA = alloc_mats(fused_a ? 49 : 40, q), LB = alloc_mats(fused_b ? 49 : 40, q),
+        P2 = alloc_mats(49, q);
     // residue products written by the leaf GEMM itself (no level between) on
     // its 256-tile kernel: the 49 products of a modulus interleave in 128-byte
     // segments, so that the post-combination reads one contiguous 49 x 128
@@ -409,7 +410,9 @@ struct Builder {
       for (int j = 0; j < 7; j++) {
         const int k = 7 * i + j;
         Mat a, b;
-        if (bb_plain(0, i) && bb_plain(0, j)) {
+        if (fused_a) {
+          a = at(LA, (long long)k * q);
+        } else if (bb_plain(0, i) && bb_plain(0, j)) {
           a = at(pb.A, (4 * quadA[i] + quadA[j]) * q);
         } else {
           a = at(LA, (long long)ka++ * q);
@@ -423,7 +426,7 @@ struct Builder {
         }
         next[k] = {a, b, {P2.p + k * pstep, P2.ps}, cseg};
       }
-    if (ka != 40 || kb != (fused_b ? 0 : 40)) broken = true;
+    if (ka != (fused_a ? 0 : 40) || kb != (fused_b ? 0 : 40)) broken = true;
     if (residue) ensure_consts();
     if (fused_b) {
       BBSplitB sp{};
@@ -436,7 +439,19 @@ struct Builder {
       sp.nrec2 = crt->nrec - 2;
       push(STEP_BB_SPLIT_B, 1, 1, q, 0, add_table(&sp, sizeof(sp)));
     }
-    push(STEP_BB_PRE, residue, 1, q, 0, add_table(&pre, sizeof(pre)), fused_b ? 1 : 0);
+    if (fused_a) {
+      BBSplitA sp{};
+      sp.A = crt->A;
+      sp.out = reinterpret_cast<int8_t *>(LA.p);
+      sp.sout = LA.ps;
+      sp.sa = crt->sa;
+      sp.sub = sub;
+      sp.m0 = crt->m;
+      sp.nrec2 = crt->nrec - 2;
+      push(STEP_BB_SPLIT_A, 1, 1, q, crt->m, add_table(&sp, sizeof(sp)));
+    } else {
+      push(STEP_BB_PRE, residue, 1, q, 0, add_table(&pre, sizeof(pre)), fused_b ? 1 : 0);
+    }
     rec(next, sub, nrec - 2, depth + 2);
     BBPost post{};
     post.q = P2.p;
@@ -488,10 +503,12 @@ struct Builder {
     std::vector<size_t> mats;
     // two fused 'b' levels at the top: the reconstruction is fused with the
     // combinations; with an even leaf size B's split is fused with its
-    // pre-additions too (no residue planes of C and B).  A's split stays a
-    // pass of its own: its leaves are transposed to row-major on the way.
+    // pre-additions too, and with leaves a multiple of 128 A's split as well
+    // (no residue planes of C, B and A).  Otherwise A's split is a pass of
+    // its own, which transposes its leaves to row-major on the way.
     const bool top2 = items.size() == 1 && fusable(1, n, d.nrec, 0) && (d.bs == 1 || d.bs == m);
     const bool fused_b = top2 && m % 2 == 0;
+    const bool fused_a = fused_b && m % 128 == 0;
     for (const BatchItem *it : items) {
       CrtMats M{};
       M.A = it->A;
@@ -509,7 +526,8 @@ struct Builder {
       M.pa = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk * M.chunks));
       M.pb = reinterpret_cast<double *>(base + alloc(sizeof(double) * 2 * n * nblk));
       M.fused_b = fused_b;
-      Mat RA = alloc_mats(1, n2);
+      M.fused_a = fused_a;
+      Mat RA = fused_a ? Mat{} : alloc_mats(1, n2);
       Mat RB = fused_b ? Mat{} : alloc_mats(1, n2);
       Mat RC = top2 ? Mat{} : alloc_mats(1, n2);
       M.ra = reinterpret_cast<int8_t *>(RA.p);
@@ -527,7 +545,7 @@ struct Builder {
       push(STEP_CRT_SCALE, moduli, 1, n, d.bs, mats[i]);
       steps().back().aux = h0 + i;
     }
-    for (size_t i = 0; i < items.size(); i++) {
+    for (size_t i = 0; i < items.size() && !fused_a; i++) {
       push(STEP_CRT_SPLIT, moduli, 1, n, d.bs, mats[i]);
       steps().back().aux = h0 + i;
     }
@@ -576,6 +594,9 @@ hipError_t run_one(const Plan &plan, const Step &st, hipStream_t s) {
                            plan.oz_moduli, kc, s, st.vec2 != 0);
     case STEP_BB_SPLIT_B:
       return launch_bb_split_b(reinterpret_cast<const BBSplitB *>(tb + st.table), st.n, kc, s);
+    case STEP_BB_SPLIT_A:
+      return launch_bb_split_a(reinterpret_cast<const BBSplitA *>(tb + st.table), st.n, st.bs,
+                               kc, s);
 
     case STEP_BB_POST:
       return launch_bb_post(reinterpret_cast<const BBPost *>(tb + st.table), st.n, st.op != 0,
@@ -620,6 +641,7 @@ int timer_of(const Step &st) {
     case STEP_OZ_SPLIT:
     case STEP_CRT_SCALE:
     case STEP_CRT_SPLIT:
+    case STEP_BB_SPLIT_A:
     case STEP_BB_SPLIT_B: return CAPS_TIMER_SPLIT;
     case STEP_OZ_RECON:
     case STEP_CRT_RECON: return CAPS_TIMER_RECON;

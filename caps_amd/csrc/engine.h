@@ -55,6 +55,7 @@ enum StepKind : int {
   STEP_MULTI_GEMM = 16,  // FP64 leaf GEMMs, bs = block tile
   STEP_MULTI_RGEMM = 17, // INT8 residue leaf GEMMs (128 x 128 tiles)
   STEP_MULTI_CRT = 18,   // residue scale / split / reconstruct kernel op (CrtKernel), vec2 = flag
+  STEP_BB_SPLIT_A = 19,  // residue mode: A's split fused with its two top 'b' levels, bs = m0
 };
 
 struct Step {

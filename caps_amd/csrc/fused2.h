@@ -58,6 +58,20 @@ struct BBSplitB {
   int nrec2;              // quadrant levels inside a depth-2 sub-block
 };
 
+// residue mode at the top of the recursion: the A side of the pre-additions
+// fused with A's split.  FP64 A (column-major leaves) in, the residues of all
+// 49 depth-2 operands out as row-major leaves (the layout the leaf GEMM reads
+// A in; the plain operands too).
+struct BBSplitA {
+  const double *A;        // 16 depth-2 sub-blocks of q elements, (a, b) at element (4a + b) * q
+  int8_t *out;            // operand (i, j) of modulus l at out + l * sout + (7 i + j) * q
+  long long sout;         // plane stride (bytes)
+  const int *sa;          // scale exponents of A's global rows
+  int sub;                // depth-2 sub-block size (n / 4)
+  int m0;                 // leaf size (a multiple of 128)
+  int nrec2;              // quadrant levels inside a depth-2 sub-block
+};
+
 // FP64: q % 2 == 0; residue: q % 4 == 0.  a_only (residue): side B is
 // produced by launch_bb_split_b instead.
 hipError_t launch_bb_pre(const BBPre *d_P, long long q, bool residue, int nmod,
@@ -65,6 +79,9 @@ hipError_t launch_bb_pre(const BBPre *d_P, long long q, bool residue, int nmod,
 // q % 4 == 0, q < 2^31, m0 even
 hipError_t launch_bb_split_b(const BBSplitB *d_P, long long q, const OzakiConsts *d_consts,
                              hipStream_t s);
+// m0 % 128 == 0 (host copy of the table's m0), q < 2^31
+hipError_t launch_bb_split_a(const BBSplitA *d_P, long long q, int m0,
+                             const OzakiConsts *d_consts, hipStream_t s);
 hipError_t launch_bb_post(const BBPost *d_P, long long q, bool residue, int npieces,
                           const OzakiConsts *d_consts, hipStream_t s);
 

@@ -11,10 +11,13 @@
 // residues every step is exact modulo m_l.
 //
 // HBM-bound: per position, FP64 pre reads 16 x 8 B and writes 40 x 8 B per
-// side; FP64 post reads 49 x 8 B and writes 16 x 8 B.  The residue post is
-// fused with the CRT reconstruction: 49 x nmod bytes in, 16 FP64 results out
-// (its VALU work, about 5,000 lane instructions per position for 14 moduli,
-// is close to its memory time).
+// side; FP64 post reads 49 x 8 B and writes 16 x 8 B.  At the top of a
+// residue-domain multiply the splits of B and A are fused with their
+// pre-additions (16 FP64 values in, the residues of all 49 operands out; A's
+// transposed to row-major leaves on the way), and the residue post with the
+// CRT reconstruction: 49 x nmod bytes in, 16 FP64 results out (its VALU work,
+// about 4,900 lane instructions per position for 14 moduli, is close to its
+// memory time).
 #include "ew_ops.h"
 #include "fused2.h"
 #include "ozaki_dev.h"
@@ -32,6 +35,8 @@ typedef const __attribute__((address_space(1))) unsigned gcuint;
 typedef __attribute__((address_space(1))) unsigned guint;
 typedef __attribute__((address_space(1))) double gdouble;
 typedef const __attribute__((address_space(1))) int gcint;
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) v4u gv4u;
 
 // level-1 operands at one sub-position from the four quadrant values (order
 // 11, 21, 12, 22) -> T1..T7 (A side) or S1..S7 (B side)
@@ -259,6 +264,91 @@ __global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restric
   }
 }
 
+// ------------------------------------------------ residue pre of A fused with the split
+// The leaf GEMM reads A's operands as row-major leaves, and A's leaves are
+// column-major, so the pass transposes on the way.  Block: a tile of 16 rows x
+// 128 columns of one leaf in all 16 depth-2 sub-blocks; thread: 2 consecutive
+// columns of one row, lanes along the rows (each 8-byte load of a wave reads
+// four whole 128-byte column runs).  The FP64 inputs are scaled by their
+// global rows' exponents and truncated once, as the separate split does; per
+// modulus the 49 operand bytes of the tile's 2,048 positions go through LDS
+// ([operand][row][column], rows padded to 144 bytes, a multiple of the
+// 16-byte reads) and leave as whole 128-byte rows of the row-major leaves
+// (64-byte pieces, 16 x 64 tiles, took 1.15 against 0.82 ms; 8-byte pieces
+// instead of 16, 0.83 against 0.81 ms).
+// Per position: 16 x 8 bytes in, 49 x nmod bytes out.
+constexpr int SA_R = 16, SA_C = 128, SA_T = 1024, SA_PITCH = 144;
+static_assert(SA_T == SA_R * SA_C / 2, "two positions per thread");
+
+__global__ __launch_bounds__(SA_T) void bb_split_a_kernel(const BBSplitA *__restrict__ P,
+                                                          long long q,
+                                                          const OzakiConsts *__restrict__ kc) {
+  __shared__ __align__(16) unsigned char S[49 * SA_R * SA_PITCH];
+  const int nmod = kc->nmod;
+  const int m0 = P->m0, sub = P->sub;
+  const long long mm0 = (long long)m0 * m0;
+  const int tr = m0 / SA_R, tc = m0 / SA_C;  // row and column tiles per leaf
+  const unsigned L2 = blockIdx.x / (unsigned)(tr * tc), w = blockIdx.x % (unsigned)(tr * tc);
+  const int r0 = (int)(w % tr) * SA_R, c0 = (int)(w / tr) * SA_C;
+  int bi, bj;
+  leaf_coords((int)L2, P->nrec2, bi, bj);
+  const int t = threadIdx.x, r = t & (SA_R - 1), c = 2 * (t / SA_R);
+  const int ri = bi * m0 + r0 + r;  // row within a depth-2 sub-block
+  int sr[4];  // exponent of row half (a & 1, b & 1) as index 2 (a & 1) + (b & 1)
+#pragma unroll
+  for (int h = 0; h < 4; h++) sr[h] = P->sa[(h >> 1) * 2 * sub + (h & 1) * sub + ri];
+  const double *src = P->A + L2 * mm0 + (long long)(c0 + c) * m0 + r0 + r;
+  double x[16][2];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const double *pk = src + (long long)k * q;
+    const int s = sr[2 * ((k >> 2) & 1) + (k & 1)];
+    x[k][0] = trunc(ldexp(pk[0], s));
+    x[k][1] = trunc(ldexp(pk[m0], s));
+  }
+  unsigned char *sw = S + r * SA_PITCH + c;
+  for (int l = 0; l < nmod; l++) {
+    const int mi = kc->m[l];
+    const double inv = kc->inv_m[l];
+    const int c22 = kc->c22[l];
+    int l1[2][7][4];  // [column][level-1 operand i][sub-position b]
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        int qv[4], o[7];
+#pragma unroll
+        for (int a = 0; a < 4; a++) qv[a] = sym_residue(x[4 * a + b][h], mi, inv, c22);
+        level_ops(0, qv, o);
+#pragma unroll
+        for (int i = 0; i < 7; i++) l1[h][i][b] = o[i];
+      }
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+      int o0[7], o1[7];
+      level_ops(0, l1[0][i], o0);
+      level_ops(0, l1[1][i], o1);
+#pragma unroll
+      for (int j = 0; j < 7; j++) {
+        const bool plain = bb_plain(0, i) && bb_plain(0, j);
+        const int v0 = plain ? o0[j] : mod_k(o0[j], c22, mi);
+        const int v1 = plain ? o1[j] : mod_k(o1[j], c22, mi);
+        *reinterpret_cast<unsigned short *>(sw + (7 * i + j) * SA_R * SA_PITCH) =
+            (unsigned short)((v0 & 0xff) | ((v1 & 0xff) << 8));
+      }
+    }
+    __syncthreads();
+    // 49 operands x 16 rows x 8 pieces of 16 bytes
+    int8_t *ob = P->out + l * P->sout + L2 * mm0 + (long long)r0 * m0 + c0;
+    for (int idx = t; idx < 49 * SA_R * 8; idx += SA_T) {
+      const int ch = idx & 7, rr = (idx >> 3) & (SA_R - 1), k = idx >> 7;
+      const v4u v = *reinterpret_cast<const v4u *>(S + (k * SA_R + rr) * SA_PITCH + 16 * ch);
+      __builtin_nontemporal_store(v, (gv4u *)(ob + k * q + (long long)rr * m0 + 16 * ch));
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- residue post + CRT
 // One block per PPOS consecutive positions of the depth-2 sub-blocks, two
 // phases through LDS (each product is read in PPOS-byte segments: 128 keeps
@@ -404,6 +494,15 @@ hipError_t launch_bb_split_b(const BBSplitB *P, long long q, const OzakiConsts *
                              hipStream_t s) {
   if (q % 4 || q >= (1LL << 31)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(bb_split_b_kernel, dim3(blocks_for(q / 2, 1)), dim3(T), 0, s, P, q, kc);
+  return hipGetLastError();
+}
+
+hipError_t launch_bb_split_a(const BBSplitA *P, long long q, int m0, const OzakiConsts *kc,
+                             hipStream_t s) {
+  if (m0 <= 0 || m0 % SA_C || q % ((long long)m0 * m0) || q >= (1LL << 31))
+    return hipErrorInvalidValue;
+  const long long blocks = q / (SA_R * SA_C);  // one per 16 x 128 tile
+  hipLaunchKernelGGL(bb_split_a_kernel, dim3((unsigned)blocks), dim3(SA_T), 0, s, P, q, kc);
   return hipGetLastError();
 }
 

@@ -60,6 +60,7 @@ def _setup():
     (512, 1, 1, ["b", "d"], 14),       # 256 x 256 leaves: the large-tile GEMM
     (384, 1, 1, ["b", "d"], 14),       # 192 x 192 leaves: whole K slices, edge rows checked
     (256, 2, 64, ["bb", "bd"], 14),    # bs = leaf size; 'bb': B's split fused with the adds
+    (512, 2, 128, ["bb", "bd"], 14),   # bs = leaf size = 128: A's split fused with the adds too
     (232, 2, 1, ["bb"], 14),           # even leaves (m = 58), not a multiple of 4
     (160, 1, 1, ["b", "d"], 14),       # m = 80: 16-byte K chunks checked
     (2 * 544, 1, 1, ["b"], 14),        # m = 544 (17 x 32), as at n = 4352, r = 3
@@ -163,7 +164,9 @@ def test_crt_large_tile_gemm_bit_exact_vs_model():
 
 
 @pytest.mark.parametrize("n,nrec,pattern", [(512, 2, "bb"), (1024, 3, "bbd"), (256, 3, "bbb"),
-                                             (56, 3, "bbb"), (512, 2, "bd")])
+                                             (56, 3, "bbb"), (512, 2, "bd"),
+                                             # leaves of 128, 4 x 4 of them per depth-2 sub-block
+                                             (2048, 4, "bbdd")])
 def test_crt_fused_levels_bit_exact(n, nrec, pattern):
     """Fused two-level residue passes with the reconstruction folded into the
     combinations: same bits as the unfused residue recursion and the model."""
