@@ -272,12 +272,16 @@ __global__ __launch_bounds__(T) void bb_split_b_kernel(const BBSplitB *__restric
 // four whole 128-byte column runs).  The FP64 inputs are scaled by their
 // global rows' exponents and truncated once, as the separate split does; per
 // modulus the 49 operand bytes of the tile's 2,048 positions go through LDS
-// ([operand][row][column], rows padded to 144 bytes, a multiple of the
-// 16-byte reads) and leave as whole 128-byte rows of the row-major leaves
-// (64-byte pieces, 16 x 64 tiles, took 1.15 against 0.82 ms; 8-byte pieces
-// instead of 16, 0.83 against 0.81 ms).
+// ([operand][row][column], 128-byte rows whose 16-byte chunk j sits in slot
+// j ^ ((row >> 1) & 7), so that the 16 rows a 16-lane cycle touches spread
+// over the 64 banks: conflict cycles 4.5e7 per launch against 5.6e7 with rows
+// padded to 144 bytes, 0.81 against 0.825 ms) and leave as whole 128-byte
+// rows of the row-major leaves (64-byte pieces, 16 x 64 tiles, took 1.15
+// against 0.82 ms; 8-byte pieces instead of 16, 0.83 against 0.81 ms).
 // Per position: 16 x 8 bytes in, 49 x nmod bytes out.
-constexpr int SA_R = 16, SA_C = 128, SA_T = 1024, SA_PITCH = 144;
+constexpr int SA_R = 16, SA_C = 128, SA_T = 1024, SA_PITCH = 128;
+
+__device__ __forceinline__ int sa_slot(int row, int chunk) { return chunk ^ ((row >> 1) & 7); }
 static_assert(SA_T == SA_R * SA_C / 2, "two positions per thread");
 
 __global__ __launch_bounds__(SA_T) void bb_split_a_kernel(const BBSplitA *__restrict__ P,
@@ -306,7 +310,7 @@ __global__ __launch_bounds__(SA_T) void bb_split_a_kernel(const BBSplitA *__rest
     x[k][0] = trunc(ldexp(pk[0], s));
     x[k][1] = trunc(ldexp(pk[m0], s));
   }
-  unsigned char *sw = S + r * SA_PITCH + c;
+  unsigned char *sw = S + r * SA_PITCH + (sa_slot(r, c >> 4) << 4) + (c & 15);
   for (int l = 0; l < nmod; l++) {
     const int mi = kc->m[l];
     const double inv = kc->inv_m[l];
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(SA_T) void bb_split_a_kernel(const BBSplitA *__rest
     int8_t *ob = P->out + l * P->sout + L2 * mm0 + (long long)r0 * m0 + c0;
     for (int idx = t; idx < 49 * SA_R * 8; idx += SA_T) {
       const int ch = idx & 7, rr = (idx >> 3) & (SA_R - 1), k = idx >> 7;
-      const v4u v = *reinterpret_cast<const v4u *>(S + (k * SA_R + rr) * SA_PITCH + 16 * ch);
+      const v4u v = *reinterpret_cast<const v4u *>(S + (k * SA_R + rr) * SA_PITCH + 16 * sa_slot(rr, ch));
       __builtin_nontemporal_store(v, (gv4u *)(ob + k * q + (long long)rr * m0 + 16 * ch));
     }
     __syncthreads();
